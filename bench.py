@@ -1,0 +1,221 @@
+"""Apply throughput of the skeletonized FMM on one MI355X.
+
+Workload (default): 3D Laplace (LAPLACE_3D), N = 1,000,000 points i.i.d. uniform
+on [0,1]^3 (numpy default_rng(0)), charges RandomState(1).randn, tol 1e-6,
+leaf_size 400, one right-hand side. A step is one apply (charges resident in
+HBM, result left in HBM). The reported value is the FP32 precision form in
+Mpoints*RHS/s; the plain FP64 form, the FP32/FP64 speed-up, relative l2 errors
+of both forms against an FP64 direct sum at 1,000 sampled targets and the build
+seconds are reported beside it.
+
+``roofline`` is for the near-field kernel (the dominant one): algorithmic
+flops per launch (interactions x flops per interaction) over its mean launch
+duration, timed with HIP events on the stream it runs on.
+``cpu_baseline`` is the padded torch-CPU restatement of the reference apply on a
+sampled fraction of every interaction list plus full interpolation passes,
+extrapolated by interaction count.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# flops per interaction (FMA = 2): dx,dy,dz (3) + r^2 (mul + 2 fma = 5) + rsqrt (1) + q*K accumulate (2)
+FLOPS_PER_INTERACTION = {"laplace_3d": 11, "laplace_2d": 11}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--leaf", type=int, default=400)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-fraction", type=float, default=0.02)
+    ap.add_argument("--ncheck", type=int, default=1000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from skelfmm_amd import hip, operators
+    from skelfmm_amd.batched import SkelFMMBatched
+    from skelfmm_amd.build import RecursiveSkeletonization
+    from skelfmm_amd.tree import BalancedTree
+
+    if not os.path.exists(hip.LIB_PATH):
+        import __graft_entry__
+
+        __graft_entry__.build()
+    hip.require_gpu()
+    dev = torch.device("cuda", 0)
+
+    N = args.n
+    X = np.random.default_rng(0).random((N, 3))
+    q = np.random.RandomState(1).randn(N)
+    op = operators.LAPLACE_3D
+
+    t0 = time.perf_counter()
+    tree = BalancedTree(X, leaf_size=args.leaf)
+    t_tree = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    fmm = RecursiveSkeletonization(tree, op, kappa=0.0, tol=args.tol)
+    fmm.skel_tree(npoints_max=500)
+    t_skel = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    fmm.setup_lists()
+    d64 = SkelFMMBatched(tree, op).build_from_copy(fmm, use_single_precision=False)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t2
+    build_s = time.perf_counter() - t0
+    d32 = SkelFMMBatched(tree, op).build_from_copy(fmm, use_single_precision=True)
+    log(f"build {build_s:.1f}s (tree {t_tree:.1f}, skel {t_skel:.1f}, lists+upload {t_setup:.1f}); "
+        f"levels {tree.nlevels}, root {fmm.root_level}, max rank {fmm.max_rank}, "
+        f"interactions/apply {d64.interactions:.3e}")
+
+    q64 = torch.as_tensor(q, dtype=torch.float64, device=dev)
+    q32 = q64.to(torch.float32)
+
+    def timed(devapply, x):
+        y = torch.empty_like(x)
+        for _ in range(args.warmup):
+            devapply.apply_device(x, out=y)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(args.steps):
+            devapply.apply_device(x, out=y)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - ts) / args.steps, y
+
+    t64, y64 = timed(d64, q64)
+    t32, y32 = timed(d32, q32)
+    log(f"apply FP64 {t64 * 1e3:.2f} ms, FP32 {t32 * 1e3:.2f} ms, ratio {t64 / t32:.2f}")
+
+    # near-field kernel alone, HIP events on the current stream
+    def p2p_time(devapply, width=1, reps=10):
+        plan = devapply._plan(width)
+        a = devapply._p2p_args(width)
+        a.q = hip.ptr(plan["qbuf"])
+        a.u = hip.ptr(plan["ubuf"])
+        st = hip.stream_handle(dev)
+        L = hip.lib()
+        import ctypes
+
+        hip.check(L.skf_p2p(ctypes.byref(a), st))
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            hip.check(L.skf_p2p(ctypes.byref(a), st))
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e-3
+
+    p64 = p2p_time(d64)
+    p32 = p2p_time(d32)
+    fpi = FLOPS_PER_INTERACTION[op.name]
+    tf64 = d64.interactions * fpi / p64 / 1e12
+    tf32 = d32.interactions * fpi / p32 / 1e12
+    log(f"near-field kernel FP64 {p64 * 1e3:.2f} ms ({tf64:.1f} TF/s), FP32 {p32 * 1e3:.2f} ms ({tf32:.1f} TF/s)")
+
+    # errors against the FP64 direct sum at sampled targets
+    from oracle.direct import direct_sum_c
+
+    idx = np.random.default_rng(0).choice(N, min(args.ncheck, N), replace=False)
+    ref = direct_sum_c(op.name, X[idx], X, q)
+    y64h = y64.cpu().numpy()
+    y32h = y32.cpu().numpy().astype(np.float64)
+    e64 = float(np.linalg.norm(y64h[idx] - ref) / np.linalg.norm(ref))
+    e32 = float(np.linalg.norm(y32h[idx] - ref) / np.linalg.norm(ref))
+    log(f"rel-l2 vs direct sum at {idx.size} targets: FP64 {e64:.3e}, FP32 {e32:.3e}")
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.batched_apply import BatchedApplyCPU
+
+        ncores = min(16, len(os.sched_getaffinity(0)))
+        torch.set_num_threads(ncores)
+        ba = BatchedApplyCPU(fmm, bytes_budget=10e9, pair_fraction=args.cpu_fraction)
+        ba.matvec(q[:])  # warm-up
+        best = None
+        for _ in range(2):
+            tc = time.perf_counter()
+            ba.matvec(q)
+            el = time.perf_counter() - tc
+            best = el if best is None else min(best, el)
+        frac_inter = ba.interactions
+        est = best * d64.interactions / max(frac_inter, 1)
+        cpu = {
+            "value": round(N / est / 1e6, 6),
+            "unit": "Mpoints*RHS/s",
+            "cores": ncores,
+            "kind": "port",
+            "sample": (f"padded torch-CPU FP64 restatement of the reference apply on {args.cpu_fraction:.0%} of the "
+                       f"pairs of every interaction list ({frac_inter:.3e} interactions) plus full interpolation "
+                       f"passes, {best:.1f}s, extrapolated to {d64.interactions:.3e} interactions"),
+        }
+        log(f"cpu baseline: {cpu}")
+
+    value = N / t32 / 1e6
+    line = {
+        "metric": "Mpoints*RHS/s per apply, FP32 precision form (FP64 form and build s beside it)",
+        "value": round(value, 3),
+        "unit": "Mpoints*RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t32 * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: uniform [0,1]^3 points default_rng(0), RandomState(1) N(0,1) charges",
+        "config": {"workload": "laplace_3d_uniform_cube", "N": N, "tol": args.tol, "leaf_size": args.leaf,
+                   "nrhs": 1, "root_level": int(fmm.root_level), "nlevels": int(tree.nlevels),
+                   "interactions_per_apply": int(d64.interactions)},
+        "fp64": {"value": round(N / t64 / 1e6, 3), "ms_per_step": round(t64 * 1e3, 4), "rel_l2_err": e64},
+        "fp32_rel_l2_err": e32,
+        "fp32_over_fp64_speedup": round(t64 / t32, 3),
+        "build_s": round(build_s, 2),
+        "build_split_s": {"tree": round(t_tree, 2), "skel": round(t_skel, 2), "lists_upload": round(t_setup, 2)},
+        "roofline": {
+            "bound": "mfma",
+            "achieved": round(tf32, 3),
+            "peak": PEAK_TFLOPS["fp32"],
+            "unit": "TFLOP/s",
+            "frac": round(tf32 / PEAK_TFLOPS["fp32"], 4),
+            "traffic": None,
+            "kernel": "p2p_kernel<float, LAPLACE_3D> (near field, VALU/transcendental bound)",
+            "kernel_ms": round(p32 * 1e3, 4),
+            "flops_per_interaction": fpi,
+            "fp64_kernel": {"achieved": round(tf64, 3), "peak": PEAK_TFLOPS["fp64"],
+                            "frac": round(tf64 / PEAK_TFLOPS["fp64"], 4), "kernel_ms": round(p64 * 1e3, 4)},
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,179 @@
+/*
+ * skelfmm_hip.h -- C ABI of the MI355X (gfx950) apply library libskelfmm_hip.so.
+ *
+ * All device buffers are owned by the caller (allocated through PyTorch's
+ * caching allocator or hipMalloc) and passed as plain pointers. The library
+ * allocates nothing on the apply path. Every entry point returns 0 on success
+ * and a nonzero skf_status otherwise; skf_last_error() then holds a message.
+ * Launches are asynchronous on the stream passed in (a hipStream_t cast to
+ * void*, NULL = default stream); the library never synchronizes the host.
+ *
+ * Reference interfaces replaced (files of annayesy/skelFMM):
+ *   skf_apply        SkelFMMBatched.matvec            skelfmm/batched_fmm.py:539-564
+ *                    (get_qorig 441-447, compute_qskel 450-467, addto_uvec 496-511,
+ *                     subtract_from_uskel 514-534, resultvec_from_uskel 469-483,
+ *                     get_result 486-490)
+ *   skf_p2p          translation()/translation_helper()/kernel_apply()
+ *                                                     skelfmm/batched_fmm.py:24-44,181-372
+ *                    and the operator kernels         skelfmm/util_batched.py:226-656
+ *   skf_up_level     apply_proj_leaves/levtree(up) + addto_skel_lev(up)
+ *                                                     skelfmm/batched_fmm.py:381-438
+ *   skf_down_level   addto_skel_lev(down) + apply_proj_levtree/leaves(down)
+ *                                                     skelfmm/batched_fmm.py:381-438,469-483
+ *   skf_gather_charges / skf_scatter_result
+ *                    get_qorig / get_result           skelfmm/batched_fmm.py:441-447,486-490
+ *   skf_p2p_blocks   KernelOperator.torch_apply on (B, m, d) blocks
+ *                                                     skelfmm/operators.py:69-89,
+ *                                                     skelfmm/util_batched.py:240-656
+ */
+#ifndef SKELFMM_HIP_H
+#define SKELFMM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* operator codes (skelfmm_amd.operators.KID_*) */
+enum skf_kernel {
+  SKF_LAPLACE_2D = 0,
+  SKF_LAPLACE_3D = 1,
+  SKF_HELMHOLTZ_2D = 2,
+  SKF_HELMHOLTZ_3D = 3,
+  SKF_LAPLACE_DL_3D = 4,
+  SKF_LAPLACE_ADL_3D = 5,
+  SKF_HELMHOLTZ_DL_3D = 6,
+  SKF_HELMHOLTZ_ADL_3D = 7
+};
+
+enum skf_status {
+  SKF_OK = 0,
+  SKF_ERR_ARG = 1,      /* bad argument / shape */
+  SKF_ERR_HIP = 2,      /* a HIP runtime call failed */
+  SKF_ERR_UNSUPPORTED = 3
+};
+
+/* source-entry flag bits */
+#define SKF_SRC_NEG 1  /* subtract this source segment's contribution */
+#define SKF_SRC_SELF 2 /* the segment is the target segment itself: skip i == j */
+
+/*
+ * Near-field work: target segments ("groups") of consecutive point slots, each
+ * with a list of source segments. Every work item covers up to 64 targets of
+ * one group and a contiguous range of that group's source entries.
+ *
+ *   group g : points pt[g] .. pt[g]+n[g]-1, results accumulated into
+ *             u[out[g] + i] (i < n[g]); box id box[g] (precision form centre)
+ *   entry s : points pt[s] .. pt[s]+n[s]-1 with charges q[qoff[s] + j];
+ *             flags SKF_SRC_*; box id box[s]
+ *   item    : int4 {group, first target, first entry, end entry}
+ *
+ * Vectors hold `width` real scalars per slot ([slot][width]); for complex
+ * kernels `width` = 2*nrhs (interleaved re/im). Real kernels treat complex
+ * vectors as 2*nrhs real channels.
+ */
+typedef struct skf_p2p_args {
+  int kernel;         /* enum skf_kernel */
+  int precision;      /* 0: FP64 points/vectors, 1: FP32 (box-centred points, FP64 centres) */
+  int width;          /* real scalars per vector slot */
+  double kappa;       /* wavenumber (Helmholtz), 0 otherwise */
+  const void* px;     /* point coordinates, SoA, Real[nslots] each */
+  const void* py;
+  const void* pz;     /* NULL in 2D */
+  const void* nx;     /* normals (double-layer kernels) or NULL */
+  const void* ny;
+  const void* nz;
+  const double* centers; /* (nbox, 3) box centres, precision form only */
+  const void* q;      /* charges, Real[...][width] */
+  void* u;            /* potentials, Real[...][width], accumulated */
+  int ngroups;
+  const int* g_pt;
+  const int* g_n;
+  const int* g_out;
+  const int* g_box;
+  int nentries;
+  const int* e_pt;
+  const int* e_n;
+  const int* e_q;
+  const int* e_box;
+  const int* e_flags;
+  int nitems;
+  const int* items;   /* int4 per item */
+} skf_p2p_args;
+
+/* one level of the upward or downward interpolation pass */
+typedef struct skf_level_args {
+  int nboxes;                 /* boxes of the level in the tables below */
+  const int* box_off;         /* first slot of the box */
+  const int* box_bs;          /* candidate count */
+  const int* box_rank;        /* skeleton count */
+  const long long* box_toff;  /* element offset of T (rank x (bs-rank), row-major) */
+  int ngather;                /* upward: q[dst] = qskel[src]; downward: uskel[dst] += u[src], u[dst] += uskel[dst] */
+  const int* gather_dst;
+  const int* gather_src;
+  int nitems;                 /* int2 {box index, first row (up) / column (down)} */
+  const int* items;
+} skf_level_args;
+
+typedef struct skf_plan_desc {
+  int precision;              /* 0 FP64, 1 FP32 */
+  int complex_t;              /* T entries complex (Helmholtz kernels) */
+  int width;                  /* real scalars per slot */
+  int nslots;                 /* total candidate slots */
+  int npoints;                /* N */
+  const int* leaf_slot;       /* N entries: slot of the user's point i ... */
+  const int* leaf_user;       /* ... and its user index */
+  const void* T;              /* interpolation matrices, Real (or complex Real pairs) */
+  void* qbuf;                 /* Real[2*nslots*width]: q_vec then q_skel */
+  void* ubuf;                 /* Real[2*nslots*width]: u_vec then u_skel */
+  int nup;                    /* upward stages, deepest level first (root gather last) */
+  const skf_level_args* up;
+  int ndown;                  /* downward stages, shallowest skeletonized level first */
+  const skf_level_args* down;
+  skf_p2p_args p2p;           /* q/u inside are ignored: qbuf/ubuf are used */
+} skf_plan_desc;
+
+typedef struct skf_plan skf_plan;
+
+const char* skf_last_error(void);
+int skf_version(void);
+
+/* zero `n` real scalars of the given precision */
+int skf_fill_zero(void* ptr, long long n, int precision, void* stream);
+
+/* q_vec[leaf_slot[i]][:] = x[leaf_user[i]][:] ; y[leaf_user[i]][:] = u_vec[leaf_slot[i]][:] */
+int skf_gather_charges(const void* x, void* qvec, const int* leaf_slot, const int* leaf_user, int n,
+                       int width, int precision, void* stream);
+int skf_scatter_result(const void* uvec, void* y, const int* leaf_slot, const int* leaf_user, int n,
+                       int width, int precision, void* stream);
+
+/* near-field translations; q and u as in skf_p2p_args */
+int skf_p2p(const skf_p2p_args* a, void* stream);
+
+/* one upward level: gather children's skeleton charges, then qskel = q[:r] + T q[r:bs] */
+int skf_up_level(const skf_level_args* L, const void* T, int complex_t, void* qvec, void* qskel, int width,
+                 int precision, void* stream);
+/* one downward level: uskel += parent's u, u[:r] += uskel, u[r:bs] += T^T uskel */
+int skf_down_level(const skf_level_args* L, const void* T, int complex_t, void* uvec, void* uskel, int width,
+                   int precision, void* stream);
+
+/* whole apply: y = K~ x (x, y in the caller's point order, [N][width]) */
+int skf_plan_create(const skf_plan_desc* d, skf_plan** out);
+int skf_plan_destroy(skf_plan* p);
+int skf_apply(skf_plan* p, const void* x, void* y, void* stream);
+
+/*
+ * Dense blocks: u[b, i, :] (+)= sum_j K(xt[b,i], xs[b,j]) q[b, j, :] for B blocks of
+ * m targets and n sources, AoS coordinates (B, m, 3|2), normals (B, m|n, 3) or NULL.
+ * self_interaction zeroes i == j. accumulate = 0 overwrites u.
+ */
+int skf_p2p_blocks(int kernel, int precision, int width, double kappa, int nblocks, int m, int n, int ndim,
+                   const void* xt, const void* xs, const void* nt, const void* ns, const void* q, void* u,
+                   int self_interaction, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SKELFMM_HIP_H */

@@ -1,0 +1,844 @@
+// skelfmm_hip.hip -- gfx950 kernels and C ABI for the skeletonized FMM apply.
+//
+// Apply = charge gather -> upward interpolation (per level) -> near-field
+// translations (one launch over every interaction list) -> downward
+// interpolation (per level) -> result scatter. The algebra follows
+// skelfmm/batched_fmm.py:539-564 (and the serial statement in
+// skelfmm/recursive_skel.py:784-1084); the data layout is ragged (no
+// (nboxes, max_bs) padding) and every stage runs on the GPU.
+//
+// Build: hipcc --offload-arch=gfx950 -O3 -fPIC -shared -std=c++17
+//        -Xclang -target-feature -Xclang -packed-fp32-ops -munsafe-fp-atomics
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "skelfmm_hip.h"
+
+// ------------------------------------------------------------------ errors
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define SKF_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) return set_err(SKF_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define SKF_LAUNCHED()                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = hipGetLastError();                                                  \
+    if (e_ != hipSuccess) return set_err(SKF_ERR_HIP, "kernel launch: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+extern "C" const char* skf_last_error(void) { return g_err; }
+extern "C" int skf_version(void) { return 1; }
+
+// ---------------------------------------------------------- kernel algebra
+// Closed forms of skelfmm/util_batched.py (torch path): distances are clamped
+// from below (r^2 >= 1e-15 for the cdist-based kernels, >= 1e-30 for the
+// Laplace double layers in FP64), self pairs (i == j inside a self segment)
+// contribute zero. The FP32 form clamps r^2 >= 1e-20 for every kernel.
+
+template <typename R>
+struct Cplx {
+  R re, im;
+};
+
+template <typename R>
+struct Mth;
+
+template <>
+struct Mth<double> {
+  static constexpr double clamp_cd = 1e-15;
+  static constexpr double clamp_dl = 1e-30;
+  __device__ static inline double rsq(double x) { return rsqrt(x); }
+  __device__ static inline double lg(double x) { return log(x); }
+  __device__ static inline void sc(double x, double* s, double* c) { sincos(x, s, c); }
+  __device__ static inline double bj0(double x) { return j0(x); }
+  __device__ static inline double by0(double x) { return y0(x); }
+};
+
+template <>
+struct Mth<float> {
+  static constexpr float clamp_cd = 1e-20f;
+  static constexpr float clamp_dl = 1e-20f;
+  __device__ static inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+  __device__ static inline float lg(float x) { return __logf(x); }
+  __device__ static inline void sc(float x, float* s, float* c) { sincosf(x, s, c); }
+  __device__ static inline float bj0(float x) { return j0f(x); }
+  __device__ static inline float by0(float x) { return y0f(x); }
+};
+
+// Traits per operator: spatial dimension, complex value, where normals live.
+template <int OP>
+struct OpT;
+template <> struct OpT<SKF_LAPLACE_2D> { static constexpr int dim = 2, cplx = 0, geom = 0; };
+template <> struct OpT<SKF_LAPLACE_3D> { static constexpr int dim = 3, cplx = 0, geom = 0; };
+template <> struct OpT<SKF_HELMHOLTZ_2D> { static constexpr int dim = 2, cplx = 1, geom = 0; };
+template <> struct OpT<SKF_HELMHOLTZ_3D> { static constexpr int dim = 3, cplx = 1, geom = 0; };
+template <> struct OpT<SKF_LAPLACE_DL_3D> { static constexpr int dim = 3, cplx = 0, geom = 1; };
+template <> struct OpT<SKF_LAPLACE_ADL_3D> { static constexpr int dim = 3, cplx = 0, geom = 2; };
+template <> struct OpT<SKF_HELMHOLTZ_DL_3D> { static constexpr int dim = 3, cplx = 1, geom = 1; };
+template <> struct OpT<SKF_HELMHOLTZ_ADL_3D> { static constexpr int dim = 3, cplx = 1, geom = 2; };
+
+// K(x, y) for dx = x - y and the normal dot product rn = dx . n (source or target normal).
+template <typename R, int OP>
+__device__ inline Cplx<R> kernel_value(R dx, R dy, R dz, R rn, R kappa) {
+  using M = Mth<R>;
+  constexpr R inv4pi = R(0.079577471545947667884441881686257181);
+  Cplx<R> k;
+  R r2 = dx * dx + dy * dy;
+  if (OpT<OP>::dim == 3) r2 += dz * dz;
+  if (OP == SKF_LAPLACE_2D) {
+    k.re = R(0.5) * M::lg(fmax(r2, M::clamp_cd));
+    k.im = 0;
+  } else if (OP == SKF_LAPLACE_3D) {
+    k.re = M::rsq(fmax(r2, M::clamp_cd));
+    k.im = 0;
+  } else if (OP == SKF_HELMHOLTZ_2D) {
+    R r = sqrt(fmax(r2, M::clamp_cd));
+    k.re = M::bj0(kappa * r);
+    k.im = M::by0(kappa * r);
+  } else if (OP == SKF_HELMHOLTZ_3D) {
+    R ir = M::rsq(fmax(r2, M::clamp_cd));
+    R r = fmax(r2, M::clamp_cd) * ir;
+    R s, c;
+    M::sc(kappa * r, &s, &c);
+    k.re = c * ir;
+    k.im = s * ir;
+  } else if (OP == SKF_LAPLACE_DL_3D || OP == SKF_LAPLACE_ADL_3D) {
+    R ir = M::rsq(fmax(r2, M::clamp_dl));
+    R v = inv4pi * rn * ir * ir * ir;
+    k.re = (OP == SKF_LAPLACE_DL_3D) ? -v : v;
+    k.im = 0;
+  } else {  // Helmholtz double layers: (1/4pi) rn e^{ikr} (ikr - 1) / r^3, sign flipped for the adjoint
+    R rc = fmax(r2, M::clamp_cd);
+    R ir = M::rsq(rc);
+    R r = rc * ir;
+    R s, c;
+    R kr = kappa * r;
+    M::sc(kr, &s, &c);
+    R f = inv4pi * rn * ir * ir * ir;
+    if (OP == SKF_HELMHOLTZ_ADL_3D) f = -f;
+    // e^{ikr}(ikr - 1) = (c + i s)(-1 + i kr) = (-c - kr s) + i (kr c - s)
+    k.re = f * (-c - kr * s);
+    k.im = f * (kr * c - s);
+  }
+  return k;
+}
+
+// --------------------------------------------------------------- P2P kernel
+namespace {
+
+constexpr int kWG = 256;   // 4 wave64 per workgroup
+constexpr int kTCH = 64;   // targets per work item (one per lane)
+
+struct P2PDev {
+  const void* px;
+  const void* py;
+  const void* pz;
+  const void* nx;
+  const void* ny;
+  const void* nz;
+  const double* centers;
+  const void* q;
+  void* u;
+  const int* g_pt;
+  const int* g_n;
+  const int* g_out;
+  const int* g_box;
+  const int* e_pt;
+  const int* e_n;
+  const int* e_q;
+  const int* e_box;
+  const int* e_flags;
+  const int4* items;
+  int width;
+  int nch;  // channels (complex channels for complex kernels)
+  double kappa;
+};
+
+__device__ inline void atomic_acc(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ inline void atomic_acc(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+// Values per staged source: coordinates, optional source normal, NCH charges (x2 if complex).
+template <int OP, int NCH>
+struct Stage {
+  static constexpr int dim = OpT<OP>::dim;
+  static constexpr int cw = OpT<OP>::cplx ? 2 : 1;
+  static constexpr int ng = OpT<OP>::geom == 1 ? 3 : 0;
+  static constexpr int nraw = dim + ng + NCH * cw;
+  static constexpr int nv = (nraw + 1) & ~1;  // even -> 16-byte aligned rows
+};
+
+// One work item: up to 64 targets of one group against a range of its source entries.
+// The 4 waves of the workgroup split each 256-source tile (64 sources per wave);
+// partial sums are combined through LDS and added to the output with atomics.
+template <typename R, int OP, int NCH, bool PREC>
+__global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
+  using S = Stage<OP, NCH>;
+  constexpr int NV = S::nv;
+  constexpr int CW = S::cw;
+  constexpr int NA = NCH * CW;
+  __shared__ R tile[kWG * NV];
+  __shared__ double red[4][kTCH][NA];
+
+  const int4 it = d.items[blockIdx.x];
+  const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
+  const int ch0 = blockIdx.y * NCH;
+  const int nchv = min(NCH, d.nch - ch0);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ng = d.g_n[g];
+  const int nt = min(kTCH, ng - t0);
+  const int tloc = t0 + lane;
+  const bool tvalid = lane < nt;
+  const int tslot = d.g_pt[g] + (tvalid ? tloc : t0);
+
+  const R* px = static_cast<const R*>(d.px);
+  const R* py = static_cast<const R*>(d.py);
+  const R* pz = static_cast<const R*>(d.pz);
+  const R* qv = static_cast<const R*>(d.q);
+  R xt = px[tslot], yt = py[tslot], zt = (S::dim == 3) ? pz[tslot] : R(0);
+  R gxt = 0, gyt = 0, gzt = 0;
+  if (OpT<OP>::geom == 2) {
+    gxt = static_cast<const R*>(d.nx)[tslot];
+    gyt = static_cast<const R*>(d.ny)[tslot];
+    gzt = static_cast<const R*>(d.nz)[tslot];
+  }
+  double ctx = 0, cty = 0, ctz = 0;
+  if (PREC) {
+    const int gb = d.g_box[g];
+    ctx = d.centers[3 * gb + 0];
+    cty = d.centers[3 * gb + 1];
+    ctz = d.centers[3 * gb + 2];
+  }
+  const R kappa = R(d.kappa);
+
+  double acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = 0.0;
+
+  for (int s = sb; s < se; ++s) {
+    const int ps = d.e_pt[s], ns = d.e_n[s], qo = d.e_q[s], fl = d.e_flags[s];
+    const bool self = (fl & SKF_SRC_SELF) != 0;
+    const double sgn = (fl & SKF_SRC_NEG) ? -1.0 : 1.0;
+    // target coordinates relative to this source box (precision form)
+    R xs0 = xt, ys0 = yt, zs0 = zt;
+    if (PREC) {
+      const int eb = d.e_box[s];
+      xs0 = xt + R(ctx - d.centers[3 * eb + 0]);
+      ys0 = yt + R(cty - d.centers[3 * eb + 1]);
+      zs0 = zt + R(ctz - d.centers[3 * eb + 2]);
+    }
+    for (int base = 0; base < ns; base += kWG) {
+      // stage 256 sources (coordinates, normals, charges of this channel block)
+      {
+        const int j = base + tid;
+        R* row = &tile[tid * NV];
+        if (j < ns) {
+          const int sl = ps + j;
+          row[0] = px[sl];
+          row[1] = py[sl];
+          if (S::dim == 3) row[2] = pz[sl];
+          if (S::ng) {
+            row[S::dim + 0] = static_cast<const R*>(d.nx)[sl];
+            row[S::dim + 1] = static_cast<const R*>(d.ny)[sl];
+            row[S::dim + 2] = static_cast<const R*>(d.nz)[sl];
+          }
+          const R* qs = qv + (long long)(qo + j) * d.width + ch0 * CW;
+#pragma unroll
+          for (int c = 0; c < NA; ++c) row[S::dim + S::ng + c] = (c < nchv * CW) ? qs[c] : R(0);
+        } else {
+#pragma unroll
+          for (int c = 0; c < NV; ++c) row[c] = R(0);
+        }
+      }
+      __syncthreads();
+      const int wb = base + wave * 64;
+      const int cnt = max(0, min(64, ns - wb));
+      R part[NA];
+#pragma unroll
+      for (int a = 0; a < NA; ++a) part[a] = 0;
+      const R* tw = &tile[wave * 64 * NV];
+#pragma unroll 4
+      for (int k = 0; k < cnt; ++k) {
+        const R* row = tw + k * NV;
+        R dx = xs0 - row[0];
+        R dy = ys0 - row[1];
+        R dz = (S::dim == 3) ? zs0 - row[2] : R(0);
+        R rn = 0;
+        if (OpT<OP>::geom == 1) rn = dx * row[S::dim] + dy * row[S::dim + 1] + dz * row[S::dim + 2];
+        if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
+        Cplx<R> kv = kernel_value<R, OP>(dx, dy, dz, rn, kappa);
+        if (self && (wb + k) == tloc) {
+          kv.re = 0;
+          kv.im = 0;
+        }
+        const R* qq = row + S::dim + S::ng;
+        if (CW == 1) {
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) part[c] += kv.re * qq[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < NCH; ++c) {
+            part[2 * c] += kv.re * qq[2 * c] - kv.im * qq[2 * c + 1];
+            part[2 * c + 1] += kv.re * qq[2 * c + 1] + kv.im * qq[2 * c];
+          }
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < NA; ++a) acc[a] += sgn * double(part[a]);
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NA; ++a) red[wave][lane][a] = acc[a];
+  __syncthreads();
+  if (wave == 0 && tvalid) {
+    R* uo = static_cast<R*>(d.u) + (long long)(d.g_out[g] + tloc) * d.width + ch0 * CW;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) {
+      if (a < nchv * CW) {
+        double v = red[0][lane][a] + red[1][lane][a] + red[2][lane][a] + red[3][lane][a];
+        atomic_acc(&uo[a], R(v));
+      }
+    }
+  }
+}
+
+template <typename R, int OP, int NCH, bool PREC>
+void launch_p2p_t(const P2PDev& d, int nitems, hipStream_t st) {
+  dim3 grid(nitems, (d.nch + NCH - 1) / NCH);
+  hipLaunchKernelGGL((p2p_kernel<R, OP, NCH, PREC>), grid, dim3(kWG), 0, st, d);
+}
+
+template <typename R, bool PREC, int NCH>
+int launch_p2p_op(int op, const P2PDev& d, int nitems, hipStream_t st) {
+  switch (op) {
+    case SKF_LAPLACE_2D: launch_p2p_t<R, SKF_LAPLACE_2D, NCH, PREC>(d, nitems, st); break;
+    case SKF_LAPLACE_3D: launch_p2p_t<R, SKF_LAPLACE_3D, NCH, PREC>(d, nitems, st); break;
+    case SKF_HELMHOLTZ_2D: launch_p2p_t<R, SKF_HELMHOLTZ_2D, NCH, PREC>(d, nitems, st); break;
+    case SKF_HELMHOLTZ_3D: launch_p2p_t<R, SKF_HELMHOLTZ_3D, NCH, PREC>(d, nitems, st); break;
+    case SKF_LAPLACE_DL_3D: launch_p2p_t<R, SKF_LAPLACE_DL_3D, NCH, PREC>(d, nitems, st); break;
+    case SKF_LAPLACE_ADL_3D: launch_p2p_t<R, SKF_LAPLACE_ADL_3D, NCH, PREC>(d, nitems, st); break;
+    case SKF_HELMHOLTZ_DL_3D: launch_p2p_t<R, SKF_HELMHOLTZ_DL_3D, NCH, PREC>(d, nitems, st); break;
+    case SKF_HELMHOLTZ_ADL_3D: launch_p2p_t<R, SKF_HELMHOLTZ_ADL_3D, NCH, PREC>(d, nitems, st); break;
+    default: return set_err(SKF_ERR_ARG, "unknown kernel id %d", op);
+  }
+  SKF_LAUNCHED();
+  return SKF_OK;
+}
+
+template <typename R, bool PREC>
+int launch_p2p_prec(int op, const P2PDev& d, int nitems, hipStream_t st) {
+  if (d.nch == 1) return launch_p2p_op<R, PREC, 1>(op, d, nitems, st);
+  if (d.nch <= 2) return launch_p2p_op<R, PREC, 2>(op, d, nitems, st);
+  return launch_p2p_op<R, PREC, 4>(op, d, nitems, st);
+}
+
+bool op_is_complex(int op) {
+  return op == SKF_HELMHOLTZ_2D || op == SKF_HELMHOLTZ_3D || op == SKF_HELMHOLTZ_DL_3D ||
+         op == SKF_HELMHOLTZ_ADL_3D;
+}
+bool op_dim3(int op) { return !(op == SKF_LAPLACE_2D || op == SKF_HELMHOLTZ_2D); }
+bool op_geom(int op) { return op >= SKF_LAPLACE_DL_3D && op <= SKF_HELMHOLTZ_ADL_3D; }
+
+int run_p2p(const skf_p2p_args* a, const void* q, void* u, hipStream_t st) {
+  if (a->nitems == 0) return SKF_OK;
+  if (a->width <= 0) return set_err(SKF_ERR_ARG, "width must be positive");
+  const bool cx = op_is_complex(a->kernel);
+  if (cx && (a->width % 2)) return set_err(SKF_ERR_ARG, "complex kernel needs an even width");
+  if (!a->px || !a->py || (op_dim3(a->kernel) && !a->pz)) return set_err(SKF_ERR_ARG, "missing coordinates");
+  if (op_geom(a->kernel) && (!a->nx || !a->ny || !a->nz)) return set_err(SKF_ERR_ARG, "missing normals");
+  if (a->precision == 1 && !a->centers) return set_err(SKF_ERR_ARG, "precision form needs box centres");
+  P2PDev d;
+  d.px = a->px; d.py = a->py; d.pz = a->pz;
+  d.nx = a->nx; d.ny = a->ny; d.nz = a->nz;
+  d.centers = a->centers;
+  d.q = q; d.u = u;
+  d.g_pt = a->g_pt; d.g_n = a->g_n; d.g_out = a->g_out; d.g_box = a->g_box;
+  d.e_pt = a->e_pt; d.e_n = a->e_n; d.e_q = a->e_q; d.e_box = a->e_box; d.e_flags = a->e_flags;
+  d.items = reinterpret_cast<const int4*>(a->items);
+  d.width = a->width;
+  d.nch = cx ? a->width / 2 : a->width;
+  d.kappa = a->kappa;
+  if (a->precision == 0) return launch_p2p_prec<double, false>(a->kernel, d, a->nitems, st);
+  if (a->precision == 1) return launch_p2p_prec<float, true>(a->kernel, d, a->nitems, st);
+  return set_err(SKF_ERR_ARG, "precision must be 0 or 1");
+}
+
+// ---------------------------------------------------- interpolation kernels
+template <typename R>
+__global__ void fill_zero_kernel(R* p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = R(0);
+}
+
+// dst[ddst[i]*w + c] = src[dsrc[i]*w + c]  (or dst[ddst] with user/slot maps)
+template <typename R>
+__global__ void map_copy_kernel(const R* __restrict__ src, R* __restrict__ dst, const int* __restrict__ isrc,
+                                const int* __restrict__ idst, int n, int w) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long tot = (long long)n * w;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; t < tot; t += stride) {
+    int i = int(t / w), c = int(t - (long long)i * w);
+    dst[(long long)idst[i] * w + c] = src[(long long)isrc[i] * w + c];
+  }
+}
+
+// downward parent step: uskel[dst] += u[src]; u[dst] += uskel[dst]
+template <typename R>
+__global__ void down_parent_kernel(R* __restrict__ uvec, R* __restrict__ uskel, const int* __restrict__ idst,
+                                   const int* __restrict__ isrc, int n, int w) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long tot = (long long)n * w;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; t < tot; t += stride) {
+    int i = int(t / w), c = int(t - (long long)i * w);
+    long long dd = (long long)idst[i] * w + c;
+    R us = uskel[dd] + uvec[(long long)isrc[i] * w + c];
+    uskel[dd] = us;
+    uvec[dd] += us;
+  }
+}
+
+constexpr int kRowsPerWave = 4;
+constexpr int kRowsPerItem = 4 * kRowsPerWave;
+constexpr int kColsPerItem = 256;
+constexpr int kIB = 4;  // channels per pass in the interpolation kernels
+
+template <typename R, bool CX>
+__device__ inline void cmac(R& ar, R& ai, R tr, R ti, R xr, R xi) {
+  if (CX) {
+    ar += tr * xr - ti * xi;
+    ai += tr * xi + ti * xr;
+  } else {
+    ar += tr * xr;
+  }
+}
+
+// Upward: qskel[off+j] = q[off+j] + sum_i T[j,i] q[off+r+i] for the item's rows j.
+// One wave per row: lanes stride the row of T (coalesced), a wave reduction finishes it.
+template <typename R, bool CX>
+__global__ __launch_bounds__(256) void up_gemv_kernel(const R* __restrict__ T, const int* __restrict__ box_off,
+                                                      const int* __restrict__ box_bs, const int* __restrict__ box_rank,
+                                                      const long long* __restrict__ box_toff,
+                                                      const int2* __restrict__ items, const R* __restrict__ qvec,
+                                                      R* __restrict__ qskel, int w) {
+  constexpr int CWd = CX ? 2 : 1;
+  const int2 it = items[blockIdx.x];
+  const int b = it.x;
+  const int off = box_off[b], bs = box_bs[b], r = box_rank[b];
+  const long long toff = box_toff[b];
+  const int nred = bs - r;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nchan = w / CWd;
+  for (int rr = 0; rr < kRowsPerWave; ++rr) {
+    const int j = it.y + wave * kRowsPerWave + rr;
+    if (j >= r) break;
+    const R* Trow = T + (toff + (long long)j * nred) * CWd;
+    for (int c0 = 0; c0 < nchan; c0 += kIB) {
+      R ar[kIB], ai[kIB];
+#pragma unroll
+      for (int c = 0; c < kIB; ++c) ar[c] = ai[c] = 0;
+      for (int i = lane; i < nred; i += 64) {
+        const R tr = Trow[(long long)i * CWd];
+        const R ti = CX ? Trow[(long long)i * CWd + 1] : R(0);
+        const R* x = qvec + (long long)(off + r + i) * w;
+#pragma unroll
+        for (int c = 0; c < kIB; ++c) {
+          if (c0 + c < nchan) {
+            const R xr = x[(c0 + c) * CWd];
+            const R xi = CX ? x[(c0 + c) * CWd + 1] : R(0);
+            cmac<R, CX>(ar[c], ai[c], tr, ti, xr, xi);
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < kIB; ++c) {
+        for (int o = 32; o > 0; o >>= 1) {
+          ar[c] += __shfl_xor(ar[c], o, 64);
+          if (CX) ai[c] += __shfl_xor(ai[c], o, 64);
+        }
+      }
+      if (lane == 0) {
+        const R* x = qvec + (long long)(off + j) * w;
+        R* y = qskel + (long long)(off + j) * w;
+#pragma unroll
+        for (int c = 0; c < kIB; ++c) {
+          if (c0 + c < nchan) {
+            y[(c0 + c) * CWd] = x[(c0 + c) * CWd] + ar[c];
+            if (CX) y[(c0 + c) * CWd + 1] = x[(c0 + c) * CWd + 1] + ai[c];
+          }
+        }
+      }
+    }
+  }
+}
+
+// Downward: u[off+r+i] += sum_j T[j,i] uskel[off+j] for the item's columns i (thread per column).
+template <typename R, bool CX>
+__global__ __launch_bounds__(256) void down_gemv_kernel(const R* __restrict__ T, const int* __restrict__ box_off,
+                                                        const int* __restrict__ box_bs,
+                                                        const int* __restrict__ box_rank,
+                                                        const long long* __restrict__ box_toff,
+                                                        const int2* __restrict__ items, R* __restrict__ uvec,
+                                                        const R* __restrict__ uskel, int w) {
+  constexpr int CWd = CX ? 2 : 1;
+  constexpr int JB = 128;
+  __shared__ R us[JB * kIB * CWd];
+  const int2 it = items[blockIdx.x];
+  const int b = it.x;
+  const int off = box_off[b], bs = box_bs[b], r = box_rank[b];
+  const long long toff = box_toff[b];
+  const int nred = bs - r;
+  const int i = it.y + threadIdx.x;
+  const int nchan = w / CWd;
+  for (int c0 = 0; c0 < nchan; c0 += kIB) {
+    R ar[kIB], ai[kIB];
+#pragma unroll
+    for (int c = 0; c < kIB; ++c) ar[c] = ai[c] = 0;
+    for (int j0 = 0; j0 < r; j0 += JB) {
+      const int nj = min(JB, r - j0);
+      __syncthreads();
+      for (int t = threadIdx.x; t < nj * kIB; t += blockDim.x) {
+        const int jj = t / kIB, c = t % kIB;
+        const R* x = uskel + (long long)(off + j0 + jj) * w;
+        us[(jj * kIB + c) * CWd] = (c0 + c < nchan) ? x[(c0 + c) * CWd] : R(0);
+        if (CX) us[(jj * kIB + c) * CWd + 1] = (c0 + c < nchan) ? x[(c0 + c) * CWd + 1] : R(0);
+      }
+      __syncthreads();
+      if (i < nred) {
+        for (int jj = 0; jj < nj; ++jj) {
+          const R* tp = T + (toff + (long long)(j0 + jj) * nred + i) * CWd;
+          const R tr = tp[0];
+          const R ti = CX ? tp[1] : R(0);
+#pragma unroll
+          for (int c = 0; c < kIB; ++c)
+            cmac<R, CX>(ar[c], ai[c], tr, ti, us[(jj * kIB + c) * CWd], CX ? us[(jj * kIB + c) * CWd + 1] : R(0));
+        }
+      }
+    }
+    if (i < nred) {
+      R* y = uvec + (long long)(off + r + i) * w;
+#pragma unroll
+      for (int c = 0; c < kIB; ++c) {
+        if (c0 + c < nchan) {
+          y[(c0 + c) * CWd] += ar[c];
+          if (CX) y[(c0 + c) * CWd + 1] += ai[c];
+        }
+      }
+    }
+  }
+}
+
+int grid_for(long long n) {
+  long long g = (n + 255) / 256;
+  if (g < 1) g = 1;
+  if (g > 65535) g = 65535;
+  return int(g);
+}
+
+template <typename R>
+int run_up_level(const skf_level_args* L, const void* T, int cx, void* qvec, void* qskel, int w, hipStream_t st) {
+  R* qv = static_cast<R*>(qvec);
+  R* qs = static_cast<R*>(qskel);
+  if (L->ngather > 0) {
+    hipLaunchKernelGGL((map_copy_kernel<R>), dim3(grid_for((long long)L->ngather * w)), dim3(256), 0, st, qs, qv,
+                       L->gather_src, L->gather_dst, L->ngather, w);
+    SKF_LAUNCHED();
+  }
+  if (L->nitems > 0) {
+    const int2* items = reinterpret_cast<const int2*>(L->items);
+    if (cx)
+      hipLaunchKernelGGL((up_gemv_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                         L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
+    else
+      hipLaunchKernelGGL((up_gemv_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                         L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
+    SKF_LAUNCHED();
+  }
+  return SKF_OK;
+}
+
+template <typename R>
+int run_down_level(const skf_level_args* L, const void* T, int cx, void* uvec, void* uskel, int w, hipStream_t st) {
+  R* uv = static_cast<R*>(uvec);
+  R* us = static_cast<R*>(uskel);
+  if (L->ngather > 0) {
+    hipLaunchKernelGGL((down_parent_kernel<R>), dim3(grid_for((long long)L->ngather * w)), dim3(256), 0, st, uv, us,
+                       L->gather_dst, L->gather_src, L->ngather, w);
+    SKF_LAUNCHED();
+  }
+  if (L->nitems > 0) {
+    const int2* items = reinterpret_cast<const int2*>(L->items);
+    if (cx)
+      hipLaunchKernelGGL((down_gemv_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                         L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
+    else
+      hipLaunchKernelGGL((down_gemv_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                         L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
+    SKF_LAUNCHED();
+  }
+  return SKF_OK;
+}
+
+int check_level(const skf_level_args* L) {
+  if (!L) return set_err(SKF_ERR_ARG, "null level");
+  if (L->ngather < 0 || L->nitems < 0 || L->nboxes < 0) return set_err(SKF_ERR_ARG, "negative level sizes");
+  if (L->ngather > 0 && (!L->gather_dst || !L->gather_src)) return set_err(SKF_ERR_ARG, "missing gather maps");
+  if (L->nitems > 0 && (!L->items || !L->box_off || !L->box_bs || !L->box_rank || !L->box_toff))
+    return set_err(SKF_ERR_ARG, "missing box tables");
+  return SKF_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C entries
+extern "C" int skf_fill_zero(void* ptr, long long n, int precision, void* stream) {
+  if (n <= 0) return SKF_OK;
+  if (!ptr) return set_err(SKF_ERR_ARG, "null pointer");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (precision == 0)
+    hipLaunchKernelGGL((fill_zero_kernel<double>), dim3(grid_for(n)), dim3(256), 0, st, static_cast<double*>(ptr), n);
+  else
+    hipLaunchKernelGGL((fill_zero_kernel<float>), dim3(grid_for(n)), dim3(256), 0, st, static_cast<float*>(ptr), n);
+  SKF_LAUNCHED();
+  return SKF_OK;
+}
+
+extern "C" int skf_gather_charges(const void* x, void* qvec, const int* leaf_slot, const int* leaf_user, int n,
+                                  int width, int precision, void* stream) {
+  if (n <= 0) return SKF_OK;
+  if (!x || !qvec || !leaf_slot || !leaf_user || width <= 0) return set_err(SKF_ERR_ARG, "bad gather arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (precision == 0)
+    hipLaunchKernelGGL((map_copy_kernel<double>), dim3(grid_for((long long)n * width)), dim3(256), 0, st,
+                       static_cast<const double*>(x), static_cast<double*>(qvec), leaf_user, leaf_slot, n, width);
+  else
+    hipLaunchKernelGGL((map_copy_kernel<float>), dim3(grid_for((long long)n * width)), dim3(256), 0, st,
+                       static_cast<const float*>(x), static_cast<float*>(qvec), leaf_user, leaf_slot, n, width);
+  SKF_LAUNCHED();
+  return SKF_OK;
+}
+
+extern "C" int skf_scatter_result(const void* uvec, void* y, const int* leaf_slot, const int* leaf_user, int n,
+                                  int width, int precision, void* stream) {
+  if (n <= 0) return SKF_OK;
+  if (!y || !uvec || !leaf_slot || !leaf_user || width <= 0) return set_err(SKF_ERR_ARG, "bad scatter arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (precision == 0)
+    hipLaunchKernelGGL((map_copy_kernel<double>), dim3(grid_for((long long)n * width)), dim3(256), 0, st,
+                       static_cast<const double*>(uvec), static_cast<double*>(y), leaf_slot, leaf_user, n, width);
+  else
+    hipLaunchKernelGGL((map_copy_kernel<float>), dim3(grid_for((long long)n * width)), dim3(256), 0, st,
+                       static_cast<const float*>(uvec), static_cast<float*>(y), leaf_slot, leaf_user, n, width);
+  SKF_LAUNCHED();
+  return SKF_OK;
+}
+
+extern "C" int skf_p2p(const skf_p2p_args* a, void* stream) {
+  if (!a) return set_err(SKF_ERR_ARG, "null p2p args");
+  if (!a->q || !a->u) return set_err(SKF_ERR_ARG, "null vectors");
+  return run_p2p(a, a->q, a->u, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int skf_up_level(const skf_level_args* L, const void* T, int complex_t, void* qvec, void* qskel,
+                            int width, int precision, void* stream) {
+  int rc = check_level(L);
+  if (rc) return rc;
+  if (complex_t && (width % 2)) return set_err(SKF_ERR_ARG, "complex T needs an even width");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return precision == 0 ? run_up_level<double>(L, T, complex_t, qvec, qskel, width, st)
+                        : run_up_level<float>(L, T, complex_t, qvec, qskel, width, st);
+}
+
+extern "C" int skf_down_level(const skf_level_args* L, const void* T, int complex_t, void* uvec, void* uskel,
+                              int width, int precision, void* stream) {
+  int rc = check_level(L);
+  if (rc) return rc;
+  if (complex_t && (width % 2)) return set_err(SKF_ERR_ARG, "complex T needs an even width");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return precision == 0 ? run_down_level<double>(L, T, complex_t, uvec, uskel, width, st)
+                        : run_down_level<float>(L, T, complex_t, uvec, uskel, width, st);
+}
+
+// ------------------------------------------------------------- whole apply
+struct skf_plan {
+  skf_plan_desc d;
+  std::vector<skf_level_args> up;
+  std::vector<skf_level_args> down;
+};
+
+extern "C" int skf_plan_create(const skf_plan_desc* d, skf_plan** out) {
+  if (!d || !out) return set_err(SKF_ERR_ARG, "null plan arguments");
+  if (d->precision != 0 && d->precision != 1) return set_err(SKF_ERR_ARG, "precision must be 0 or 1");
+  if (d->width <= 0 || d->nslots < 0 || d->npoints < 0) return set_err(SKF_ERR_ARG, "bad sizes");
+  if (!d->qbuf || !d->ubuf) return set_err(SKF_ERR_ARG, "missing work vectors");
+  if (d->nup < 0 || d->ndown < 0) return set_err(SKF_ERR_ARG, "bad stage counts");
+  for (int i = 0; i < d->nup; ++i) {
+    int rc = check_level(&d->up[i]);
+    if (rc) return rc;
+  }
+  for (int i = 0; i < d->ndown; ++i) {
+    int rc = check_level(&d->down[i]);
+    if (rc) return rc;
+  }
+  skf_plan* p = new (std::nothrow) skf_plan;
+  if (!p) return set_err(SKF_ERR_ARG, "out of host memory");
+  p->d = *d;
+  p->up.assign(d->up, d->up + d->nup);
+  p->down.assign(d->down, d->down + d->ndown);
+  p->d.up = p->up.data();
+  p->d.down = p->down.data();
+  *out = p;
+  return SKF_OK;
+}
+
+extern "C" int skf_plan_destroy(skf_plan* p) {
+  delete p;
+  return SKF_OK;
+}
+
+extern "C" int skf_apply(skf_plan* p, const void* x, void* y, void* stream) {
+  if (!p || !x || !y) return set_err(SKF_ERR_ARG, "null apply arguments");
+  const skf_plan_desc& d = p->d;
+  const int w = d.width, prec = d.precision;
+  const long long half = (long long)d.nslots * w;
+  const size_t rs = prec == 0 ? sizeof(double) : sizeof(float);
+  char* qb = static_cast<char*>(d.qbuf);
+  char* ub = static_cast<char*>(d.ubuf);
+  void* qvec = qb;
+  void* qskel = qb + half * rs;
+  void* uvec = ub;
+  void* uskel = ub + half * rs;
+  int rc;
+  if ((rc = skf_fill_zero(d.qbuf, 2 * half, prec, stream))) return rc;
+  if ((rc = skf_fill_zero(d.ubuf, 2 * half, prec, stream))) return rc;
+  if ((rc = skf_gather_charges(x, qvec, d.leaf_slot, d.leaf_user, d.npoints, w, prec, stream))) return rc;
+  for (int i = 0; i < d.nup; ++i)
+    if ((rc = skf_up_level(&d.up[i], d.T, d.complex_t, qvec, qskel, w, prec, stream))) return rc;
+  skf_p2p_args a = d.p2p;
+  a.precision = prec;
+  a.width = w;
+  if ((rc = run_p2p(&a, d.qbuf, d.ubuf, static_cast<hipStream_t>(stream)))) return rc;
+  for (int i = 0; i < d.ndown; ++i)
+    if ((rc = skf_down_level(&d.down[i], d.T, d.complex_t, uvec, uskel, w, prec, stream))) return rc;
+  return skf_scatter_result(uvec, y, d.leaf_slot, d.leaf_user, d.npoints, w, prec, stream);
+}
+
+// ------------------------------------------------------------ dense blocks
+namespace {
+
+// u[b, i, :] (+)= sum_j K(xt[b,i], xs[b,j]) q[b,j,:]; one workgroup per (block, 64 targets),
+// AoS inputs, no LDS staging (library seam for KernelOperator.torch_apply, not the apply hot path).
+template <typename R, int OP>
+__global__ __launch_bounds__(64) void blocks_kernel(int m, int n, int ndim, const R* __restrict__ xt,
+                                                    const R* __restrict__ xs, const R* __restrict__ gt,
+                                                    const R* __restrict__ gs, const R* __restrict__ q,
+                                                    R* __restrict__ u, int w, int self_int, R kappa) {
+  constexpr int CWd = OpT<OP>::cplx ? 2 : 1;
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= m) return;
+  const R* xi = xt + ((long long)b * m + i) * ndim;
+  const R x0 = xi[0], x1 = xi[1], x2 = ndim == 3 ? xi[2] : R(0);
+  R g0 = 0, g1 = 0, g2 = 0;
+  if (OpT<OP>::geom == 2) {
+    const R* gi = gt + ((long long)b * m + i) * 3;
+    g0 = gi[0]; g1 = gi[1]; g2 = gi[2];
+  }
+  const int nchan = w / CWd;
+  for (int c = 0; c < nchan; ++c) {
+    double ar = 0, ai = 0;
+    for (int j = 0; j < n; ++j) {
+      const R* yj = xs + ((long long)b * n + j) * ndim;
+      R dx = x0 - yj[0], dy = x1 - yj[1], dz = ndim == 3 ? x2 - yj[2] : R(0);
+      R rn = 0;
+      if (OpT<OP>::geom == 1) {
+        const R* gj = gs + ((long long)b * n + j) * 3;
+        rn = dx * gj[0] + dy * gj[1] + dz * gj[2];
+      }
+      if (OpT<OP>::geom == 2) rn = dx * g0 + dy * g1 + dz * g2;
+      Cplx<R> kv = kernel_value<R, OP>(dx, dy, dz, rn, kappa);
+      if (self_int && i == j) kv.re = kv.im = 0;
+      const R* qj = q + ((long long)b * n + j) * w + c * CWd;
+      if (CWd == 1) {
+        ar += double(kv.re * qj[0]);
+      } else {
+        ar += double(kv.re * qj[0] - kv.im * qj[1]);
+        ai += double(kv.re * qj[1] + kv.im * qj[0]);
+      }
+    }
+    R* ui = u + ((long long)b * m + i) * w + c * CWd;
+    ui[0] = R(ar);
+    if (CWd == 2) ui[1] = R(ai);
+  }
+}
+
+template <typename R>
+int launch_blocks(int op, int nblocks, int m, int n, int ndim, const void* xt, const void* xs, const void* gt,
+                  const void* gs, const void* q, void* u, int w, int self_int, double kappa, hipStream_t st) {
+  dim3 grid((m + 63) / 64, nblocks);
+  const R* a0 = static_cast<const R*>(xt);
+  const R* a1 = static_cast<const R*>(xs);
+  const R* a2 = static_cast<const R*>(gt);
+  const R* a3 = static_cast<const R*>(gs);
+  const R* a4 = static_cast<const R*>(q);
+  R* a5 = static_cast<R*>(u);
+  R k = R(kappa);
+#define SKF_BLK(OPC) \
+  hipLaunchKernelGGL((blocks_kernel<R, OPC>), grid, dim3(64), 0, st, m, n, ndim, a0, a1, a2, a3, a4, a5, w, self_int, k)
+  switch (op) {
+    case SKF_LAPLACE_2D: SKF_BLK(SKF_LAPLACE_2D); break;
+    case SKF_LAPLACE_3D: SKF_BLK(SKF_LAPLACE_3D); break;
+    case SKF_HELMHOLTZ_2D: SKF_BLK(SKF_HELMHOLTZ_2D); break;
+    case SKF_HELMHOLTZ_3D: SKF_BLK(SKF_HELMHOLTZ_3D); break;
+    case SKF_LAPLACE_DL_3D: SKF_BLK(SKF_LAPLACE_DL_3D); break;
+    case SKF_LAPLACE_ADL_3D: SKF_BLK(SKF_LAPLACE_ADL_3D); break;
+    case SKF_HELMHOLTZ_DL_3D: SKF_BLK(SKF_HELMHOLTZ_DL_3D); break;
+    case SKF_HELMHOLTZ_ADL_3D: SKF_BLK(SKF_HELMHOLTZ_ADL_3D); break;
+    default: return set_err(SKF_ERR_ARG, "unknown kernel id %d", op);
+  }
+#undef SKF_BLK
+  SKF_LAUNCHED();
+  return SKF_OK;
+}
+
+}  // namespace
+
+extern "C" int skf_p2p_blocks(int kernel, int precision, int width, double kappa, int nblocks, int m, int n,
+                              int ndim, const void* xt, const void* xs, const void* nt, const void* ns,
+                              const void* q, void* u, int self_interaction, void* stream) {
+  if (nblocks <= 0 || m <= 0) return SKF_OK;
+  if (!xt || !xs || !q || !u || width <= 0 || n < 0) return set_err(SKF_ERR_ARG, "bad block arguments");
+  if ((op_dim3(kernel) ? 3 : 2) != ndim) return set_err(SKF_ERR_ARG, "dimension does not match the kernel");
+  if (op_is_complex(kernel) && (width % 2)) return set_err(SKF_ERR_ARG, "complex kernel needs an even width");
+  if (kernel == SKF_LAPLACE_DL_3D || kernel == SKF_HELMHOLTZ_DL_3D) {
+    if (!ns) return set_err(SKF_ERR_ARG, "source normals required");
+  }
+  if (kernel == SKF_LAPLACE_ADL_3D || kernel == SKF_HELMHOLTZ_ADL_3D) {
+    if (!nt) return set_err(SKF_ERR_ARG, "target normals required");
+  }
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (precision == 0)
+    return launch_blocks<double>(kernel, nblocks, m, n, ndim, xt, xs, nt, ns, q, u, width, self_interaction, kappa, st);
+  if (precision == 1)
+    return launch_blocks<float>(kernel, nblocks, m, n, ndim, xt, xs, nt, ns, q, u, width, self_interaction, kappa, st);
+  return set_err(SKF_ERR_ARG, "precision must be 0 or 1");
+}

@@ -1,0 +1,261 @@
+"""Device layout of a skeletonized build ("box layout").
+
+Every box with candidates owns a contiguous range of *slots*; its candidate
+points sit there skeleton first, so a box's skeleton is the first ``rank``
+slots of its range. Vectors are ``[2*nslots][width]``: the first half is the
+per-box candidate vector (the reference's ``q_vec``/``u_vec``), the second half
+the skeleton vector (``q_skel``/``u_skel``) at the same slot numbers
+(``skelfmm/batched_fmm.py:450-564`` uses ``(nboxes, max_bs)`` padded tensors
+for the same four vectors).
+
+Near-field work is one list for all five interaction lists of the reference
+(``recursive_skel.py:622-712``, ``batched_fmm.py:496-534``). For a target box U:
+
+* group (U, all) accumulates into ``u_vec[U]`` the ADD terms of every list
+  (sources: all candidates of Q with ``q_vec``) and the SUB term of the
+  coarse-down list (sources: the skeleton of the finer box with ``q_skel``);
+* group (U, skel) accumulates into ``u_skel[U]`` the SUB terms of the
+  colleague lists (skeleton of Q with ``q_skel``) and of the coarse-up list
+  (all candidates of the coarse leaf with ``q_vec``).
+
+Work items cover 64 targets of one group and a run of its source entries of at
+most ``max_sources`` points; they are sorted by cost, heaviest first.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+__all__ = ["BoxLayout", "build_layout"]
+
+SRC_NEG = 1
+SRC_SELF = 2
+TARGETS_PER_ITEM = 64
+ROWS_PER_ITEM = 16
+COLS_PER_ITEM = 256
+
+
+class LevelStage:
+    """Arrays of one upward or downward interpolation stage."""
+
+    def __init__(self, level, boxes, box_off, box_bs, box_rank, box_toff, gather_dst, gather_src, items):
+        self.level = level
+        self.boxes = boxes
+        self.box_off = box_off
+        self.box_bs = box_bs
+        self.box_rank = box_rank
+        self.box_toff = box_toff
+        self.gather_dst = gather_dst
+        self.gather_src = gather_src
+        self.items = items
+
+
+class BoxLayout:
+    pass
+
+
+def _level_tables(fmm, boxes, off, toff):
+    boxes = np.asarray(boxes, dtype=np.int64)
+    return (boxes,
+            off[boxes].astype(np.int32),
+            fmm.bs_list[boxes].astype(np.int32),
+            fmm.rank_list[boxes].astype(np.int32),
+            toff[boxes].astype(np.int64))
+
+
+def build_layout(fmm, max_sources=16384):
+    """Flatten a ``RecursiveSkeletonization`` build into device-ready arrays."""
+    tree = fmm.tree
+    nb = fmm.nboxes
+    bs = fmm.bs_list.astype(np.int64)
+    rk = fmm.rank_list.astype(np.int64)
+    L = BoxLayout()
+    L.ndim = fmm.ndim
+    L.N = fmm.N
+    L.nboxes = nb
+    L.root_level = fmm.root_level
+
+    off = -np.ones(nb, dtype=np.int64)
+    has = bs > 0
+    off[has] = np.concatenate(([0], np.cumsum(bs[has])[:-1])) if has.any() else []
+    nslots = int(bs.sum())
+    L.nslots = nslots
+    L.off = off
+
+    slot_pt = np.zeros(nslots, dtype=np.int64)
+    slot_box = np.zeros(nslots, dtype=np.int64)
+    for b in np.nonzero(has)[0]:
+        slot_pt[off[b]:off[b] + bs[b]] = fmm.cand[b]
+        slot_box[off[b]:off[b] + bs[b]] = b
+    L.slot_point = slot_pt
+    L.slot_box = slot_box
+    L.points = np.ascontiguousarray(tree.XX[slot_pt].T)  # (d, nslots)
+    L.normals = None
+    if fmm.geometry is not None:
+        L.normals = np.ascontiguousarray(fmm.geometry[slot_pt].T)
+    centers = np.zeros((nb, 3))
+    centers[:, :fmm.ndim] = tree.get_boxes_centers(np.arange(nb))
+    L.centers = centers
+
+    # charge gather / result scatter over all leaf slots (every point exactly once)
+    leaves = tree.get_leaves()
+    ls, lu = [], []
+    for b in leaves:
+        ls.append(np.arange(off[b], off[b] + bs[b]))
+        lu.append(fmm.cand[b])
+    L.leaf_slot = np.concatenate(ls).astype(np.int32)
+    L.leaf_user = np.concatenate(lu).astype(np.int32)
+    assert L.leaf_slot.shape[0] == fmm.N and np.unique(L.leaf_user).shape[0] == fmm.N
+
+    # interpolation matrices, one ragged array
+    toff = np.zeros(nb, dtype=np.int64)
+    tparts = []
+    acc = 0
+    for b in range(nb):
+        p = fmm.proj[b]
+        toff[b] = acc
+        if p is not None and rk[b] > 0 and p.size > 0:
+            tparts.append(np.ascontiguousarray(p).ravel())
+            acc += p.size
+    L.T = np.concatenate(tparts) if tparts else np.zeros(1, dtype=fmm.kernel_dtype)
+    L.T = L.T.astype(fmm.kernel_dtype, copy=False)
+    L.toff = toff
+
+    # upward stages: deepest level first; the root level only gathers
+    up = []
+    for lev in range(fmm.nlevels - 1, fmm.root_level - 1, -1):
+        gboxes = fmm.gathered_boxes_level(lev)
+        gd, gs = [], []
+        for P in gboxes:
+            cs = fmm.child_src[P]
+            if cs is None or cs.shape[0] == 0:
+                continue
+            gd.append(np.arange(off[P], off[P] + bs[P]))
+            gs.append(off[cs[:, 0]] + cs[:, 1])
+        gather_dst = np.concatenate(gd).astype(np.int32) if gd else np.zeros(0, np.int32)
+        gather_src = np.concatenate(gs).astype(np.int32) if gs else np.zeros(0, np.int32)
+        sboxes = fmm.skel_boxes_level(lev)
+        sboxes = sboxes[rk[sboxes] > 0]
+        tabs = _level_tables(fmm, sboxes, off, toff)
+        items = [(i, r0) for i, b in enumerate(sboxes) for r0 in range(0, int(rk[b]), ROWS_PER_ITEM)]
+        items = np.array(items, dtype=np.int32).reshape(-1, 2)
+        up.append(LevelStage(lev, *tabs, gather_dst, gather_src, items))
+    L.up = up
+
+    # downward stages: shallowest skeletonized level first
+    pos_in_parent = {}
+    for P in range(nb):
+        cs = fmm.child_src[P]
+        if cs is not None and cs.shape[0] > 0 and bs[P] > 0:
+            for k, (c, j) in enumerate(cs):
+                pos_in_parent[(int(c), int(j))] = off[P] + k
+    down = []
+    for lev in range(fmm.root_level + 1, fmm.nlevels):
+        sboxes = fmm.skel_boxes_level(lev)
+        sboxes = sboxes[rk[sboxes] > 0]
+        gd, gs = [], []
+        for B in sboxes:
+            r = int(rk[B])
+            gd.append(np.arange(off[B], off[B] + r))
+            gs.append(np.array([pos_in_parent[(int(B), j)] for j in range(r)], dtype=np.int64))
+        gather_dst = np.concatenate(gd).astype(np.int32) if gd else np.zeros(0, np.int32)
+        gather_src = np.concatenate(gs).astype(np.int32) if gs else np.zeros(0, np.int32)
+        tabs = _level_tables(fmm, sboxes, off, toff)
+        items = [(i, c0) for i, b in enumerate(sboxes) for c0 in range(0, int(bs[b] - rk[b]), COLS_PER_ITEM)]
+        items = np.array(items, dtype=np.int32).reshape(-1, 2)
+        down.append(LevelStage(lev, *tabs, gather_dst, gather_src, items))
+    L.down = down
+
+    _near_field(L, fmm, off, bs, rk, max_sources)
+    return L
+
+
+def _near_field(L, fmm, off, bs, rk, max_sources):
+    nslots = L.nslots
+    # (target box, target kind) -> list of (source box, source kind, charge kind, flags); kind 0 = all, 1 = skel
+    add_lists = [fmm.u_from_qskel_list, fmm.u_from_q_list, fmm.tree_uskel_from_qskel_list,
+                 fmm.uskel_from_q_list, fmm.leaf_uskel_from_qskel_list]
+    rows = []  # columns: tbox, tkind, sbox, skind, qkind, neg
+    for lst in add_lists:
+        if lst.shape[0]:
+            n = lst.shape[0]
+            rows.append(np.stack([lst[:, 0], np.zeros(n, np.int64), lst[:, 1], np.zeros(n, np.int64),
+                                  np.zeros(n, np.int64), np.zeros(n, np.int64)], axis=1))
+    lst = fmm.uskel_from_qskel_list
+    if lst.shape[0]:
+        n = lst.shape[0]
+        rows.append(np.stack([lst[:, 0], np.ones(n, np.int64), lst[:, 1], np.ones(n, np.int64),
+                              np.ones(n, np.int64), np.ones(n, np.int64)], axis=1))
+    lst = fmm.uskel_from_q_list
+    if lst.shape[0]:
+        n = lst.shape[0]
+        rows.append(np.stack([lst[:, 0], np.ones(n, np.int64), lst[:, 1], np.zeros(n, np.int64),
+                              np.zeros(n, np.int64), np.ones(n, np.int64)], axis=1))
+    lst = fmm.u_from_qskel_list
+    if lst.shape[0]:
+        n = lst.shape[0]
+        rows.append(np.stack([lst[:, 0], np.zeros(n, np.int64), lst[:, 1], np.ones(n, np.int64),
+                              np.ones(n, np.int64), np.ones(n, np.int64)], axis=1))
+    R = np.concatenate(rows) if rows else np.zeros((0, 6), np.int64)
+
+    tcount = np.where(R[:, 1] == 0, bs[R[:, 0]], rk[R[:, 0]])
+    scount = np.where(R[:, 3] == 0, bs[R[:, 2]], rk[R[:, 2]])
+    keep = (tcount > 0) & (scount > 0)
+    R = R[keep]
+    scount = scount[keep]
+    order = np.lexsort((R[:, 2], R[:, 1], R[:, 0]))
+    R = R[order]
+    scount = scount[order]
+
+    e_pt = off[R[:, 2]]
+    e_n = scount
+    e_q = off[R[:, 2]] + R[:, 4] * nslots
+    self_pair = (R[:, 0] == R[:, 2]) & (R[:, 1] == R[:, 3])
+    e_flags = R[:, 5] * 1 + self_pair * 2
+    e_box = R[:, 2]
+
+    key = R[:, 0] * 2 + R[:, 1]
+    gkeys, gstart, gcount = np.unique(key, return_index=True, return_counts=True)
+    g_box = gkeys // 2
+    g_kind = gkeys % 2
+    g_pt = off[g_box]
+    g_n = np.where(g_kind == 0, bs[g_box], rk[g_box])
+    g_out = off[g_box] + g_kind * nslots
+
+    items = []
+    costs = []
+    cums = np.concatenate(([0], np.cumsum(e_n)))
+    for g in range(gkeys.shape[0]):
+        s0, s1 = int(gstart[g]), int(gstart[g] + gcount[g])
+        # split the entry run into ranges of at most max_sources points (at least one entry each)
+        ranges = []
+        a = s0
+        while a < s1:
+            b = a + 1
+            while b < s1 and cums[b + 1] - cums[a] <= max_sources:
+                b += 1
+            ranges.append((a, b))
+            a = b
+        ng = int(g_n[g])
+        for t0 in range(0, ng, TARGETS_PER_ITEM):
+            nt = min(TARGETS_PER_ITEM, ng - t0)
+            for a, b in ranges:
+                items.append((g, t0, a, b))
+                costs.append(nt * int(cums[b] - cums[a]))
+    items = np.array(items, dtype=np.int32).reshape(-1, 4)
+    costs = np.array(costs, dtype=np.int64)
+    order = np.argsort(-costs, kind="stable")
+    L.items = np.ascontiguousarray(items[order])
+    L.item_cost = costs[order]
+    L.g_pt = g_pt.astype(np.int32)
+    L.g_n = g_n.astype(np.int32)
+    L.g_out = g_out.astype(np.int32)
+    L.g_box = g_box.astype(np.int32)
+    L.e_pt = e_pt.astype(np.int32)
+    L.e_n = e_n.astype(np.int32)
+    L.e_q = e_q.astype(np.int32)
+    L.e_box = e_box.astype(np.int32)
+    L.e_flags = e_flags.astype(np.int32)
+    # exact interaction count of one apply (targets x sources over every entry)
+    tn = g_n[np.searchsorted(gkeys, key)] if key.shape[0] else np.zeros(0, np.int64)
+    L.interactions = int(np.sum(tn.astype(np.int64) * e_n.astype(np.int64)))
