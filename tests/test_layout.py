@@ -1,0 +1,125 @@
+"""Box layout and build invariants (CPU): every map the HIP kernels consume is checked here."""
+
+import numpy as np
+import pytest
+
+from conftest import annulus_points, make_build, rel_l2, sphere_points
+from layout_emulator import emulate_apply
+from oracle.batched_apply import BatchedApplyCPU
+from skelfmm_amd import operators as ops
+from skelfmm_amd.layout import SRC_NEG, SRC_SELF, build_layout
+
+
+def _builds():
+    rng = np.random.default_rng(0)
+    yield "lap2d", make_build(rng.random((3000, 2)), ops.LAPLACE_2D, leaf_size=50), 0.0
+    yield "annulus", make_build(annulus_points(4000, 3), ops.LAPLACE_2D, tol=1e-5, leaf_size=40), 0.0
+    c = np.vstack([rng.random((2500, 2)) * 0.05, rng.random((1500, 2))])
+    yield "clustered2d", make_build(c, ops.LAPLACE_2D, leaf_size=30), 0.0
+    yield "lap3d", make_build(rng.random((8000, 3)), ops.LAPLACE_3D, leaf_size=100), 0.0
+    v = sphere_points(3000, 1)
+    yield "ldl", make_build(v, ops.LAPLACE_DOUBLE_LAYER_3D, leaf_size=60, geometry=v), 0.0
+
+
+BUILDS = list(_builds())
+
+
+@pytest.mark.parametrize("case", BUILDS, ids=lambda c: c[0])
+def test_build_structure(case):
+    """test_fmm.py:111-144: root level, zero ranks at root boxes, list levels."""
+    _, f, _ = case
+    t = f.tree
+    assert f.root_level <= 2 or f.nlevels - 1 < 2
+    for b in f.root_boxes:
+        assert f.rank_list[b] == 0
+        assert t.get_box_level(b) <= f.root_level
+    for b in range(f.nboxes):
+        if t.get_box_level(b) > f.root_level:
+            assert f.rank_list[b] > 0
+            assert f.proj[b].shape == (f.rank_list[b], f.bs_list[b] - f.rank_list[b])
+    for U, Q in f.uskel_from_qskel_list:
+        assert t.get_box_level(U) == t.get_box_level(Q) > f.root_level
+    for U, C in f.uskel_from_q_list:
+        assert t.get_box_level(U) > f.root_level and t.is_leaf(C) and t.get_box_level(C) < t.get_box_level(U)
+    assert np.array_equal(f.u_from_qskel_list, f.uskel_from_q_list[:, ::-1])
+    # every point pair is covered: leaf candidates partition the points
+    allp = np.concatenate([f.cand[b] for b in t.get_leaves()])
+    assert np.array_equal(np.sort(allp), np.arange(f.N))
+
+
+@pytest.mark.parametrize("case", BUILDS, ids=lambda c: c[0])
+def test_layout_maps(case):
+    _, f, _ = case
+    L = build_layout(f)
+    # slots hold the candidates, skeleton first
+    for b in range(f.nboxes):
+        if f.bs_list[b]:
+            assert np.array_equal(L.slot_point[L.off[b]:L.off[b] + f.bs_list[b]], f.cand[b])
+    assert np.array_equal(np.sort(L.leaf_user), np.arange(f.N))
+    assert np.array_equal(L.slot_point[L.leaf_slot], L.leaf_user)
+    for S in L.up:
+        # a parent's candidate slot holds the same point as the child skeleton slot it is gathered from
+        assert np.array_equal(L.slot_point[S.gather_dst], L.slot_point[S.gather_src])
+        assert np.all(S.box_rank > 0)
+    for S in L.down:
+        assert np.array_equal(L.slot_point[S.gather_dst], L.slot_point[S.gather_src])
+        assert np.all(L.slot_box[S.gather_src] != L.slot_box[S.gather_dst])
+    assert [S.level for S in L.up] == sorted([S.level for S in L.up], reverse=True)
+    assert [S.level for S in L.down] == sorted([S.level for S in L.down])
+
+
+@pytest.mark.parametrize("case", BUILDS, ids=lambda c: c[0])
+def test_work_items_cover_every_interaction_once(case):
+    _, f, _ = case
+    L = build_layout(f, max_sources=700)
+    cover = {}
+    for g, t0, sb, se in L.items:
+        assert 0 <= t0 < L.g_n[g] and t0 % 64 == 0 and sb < se
+        for s in range(sb, se):
+            cover[(g, t0, s)] = cover.get((g, t0, s), 0) + 1
+    ngroups = L.g_pt.shape[0]
+    starts = {}
+    for (g, t0, s) in cover:
+        starts.setdefault(g, set()).add(s)
+    for g in range(ngroups):
+        chunks = list(range(0, int(L.g_n[g]), 64))
+        ents = sorted(starts[g])
+        assert ents == list(range(ents[0], ents[-1] + 1))
+        for t0 in chunks:
+            for s in ents:
+                assert cover[(g, t0, s)] == 1
+    # self entries only where a group meets its own slots, and the counts are right
+    for g in range(ngroups):
+        for s in starts[g]:
+            if L.e_flags[s] & SRC_SELF:
+                assert L.e_pt[s] == L.g_pt[g] and L.e_n[s] == L.g_n[g]
+    neg = (L.e_flags & SRC_NEG) != 0
+    nsub = f.uskel_from_qskel_list.shape[0] + 2 * f.uskel_from_q_list.shape[0]
+    assert neg.sum() <= nsub
+    assert np.all(L.item_cost[:-1] >= L.item_cost[1:])
+
+
+@pytest.mark.parametrize("case", BUILDS, ids=lambda c: c[0])
+def test_layout_emulation_matches_oracle(case):
+    """The kernels' work lists, run in numpy, reproduce the oracle's padded apply to 1e-13."""
+    _, f, k = case
+    L = build_layout(f, max_sources=900)
+    q = np.random.RandomState(4).randn(f.N)
+    y = emulate_apply(L, f.operator.name, k, q)
+    ref = BatchedApplyCPU(f, bytes_budget=1e8).matvec(q).numpy()
+    assert rel_l2(y, ref) < 1e-13
+
+
+def test_interaction_count_matches_lists():
+    f = BUILDS[0][1]
+    L = build_layout(f)
+    bs, rk = f.bs_list, f.rank_list
+    n = 0
+    for lst in (f.u_from_qskel_list, f.u_from_q_list, f.tree_uskel_from_qskel_list, f.uskel_from_q_list,
+                f.leaf_uskel_from_qskel_list):
+        n += int(np.sum(bs[lst[:, 0]] * bs[lst[:, 1]])) if lst.shape[0] else 0
+    lst = f.uskel_from_qskel_list
+    n += int(np.sum(rk[lst[:, 0]] * rk[lst[:, 1]]))
+    lst = f.uskel_from_q_list
+    n += 2 * int(np.sum(rk[lst[:, 0]] * bs[lst[:, 1]])) if lst.shape[0] else 0
+    assert L.interactions == n
