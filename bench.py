@@ -34,6 +34,10 @@ sys.path.insert(0, ROOT)
 # flops per interaction (FMA = 2): dx,dy,dz (3) + r^2 (mul + 2 fma = 5) + rsqrt (1) + q*K accumulate (2)
 FLOPS_PER_INTERACTION = {"laplace_3d": 11, "laplace_2d": 11}
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+# HBM bytes per launch of the FP32 near-field kernel on the default workload, from the two
+# rocprofv3 --pmc passes summarised in profiles/apply_laplace3d_uniform1M_hbm_traffic.md
+# (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, scale factors from a 512 MiB copy of the same run)
+P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400): 2549.1e6 + 14.2e6}
 
 
 def log(msg):
@@ -51,6 +55,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-fraction", type=float, default=0.02)
     ap.add_argument("--ncheck", type=int, default=1000)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run one 512 MiB device-to-device copy first (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -70,6 +76,13 @@ def main():
         __graft_entry__.build()
     hip.require_gpu()
     dev = torch.device("cuda", 0)
+    if args.copy_probe:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("copy probe: 536870912 bytes read + 536870912 bytes written")
 
     N = args.n
     X = np.random.default_rng(0).random((N, 3))
@@ -205,7 +218,9 @@ def main():
             "peak": PEAK_TFLOPS["fp32"],
             "unit": "TFLOP/s",
             "frac": round(tf32 / PEAK_TFLOPS["fp32"], 4),
-            "traffic": None,
+            "traffic": P2P_FP32_TRAFFIC_BYTES.get((N, args.tol, args.leaf)),
+            "traffic_unit": "bytes per launch (HBM read + write)",
+            "compute_unit": "VALU (per-pair kernel evaluation; no matrix cores at 1 RHS)",
             "kernel": "p2p_kernel<float, LAPLACE_3D> (near field, VALU/transcendental bound)",
             "kernel_ms": round(p32 * 1e3, 4),
             "flops_per_interaction": fpi,

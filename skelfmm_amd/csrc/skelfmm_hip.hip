@@ -50,7 +50,8 @@ extern "C" int skf_version(void) { return 1; }
 // Closed forms of skelfmm/util_batched.py (torch path): distances are clamped
 // from below (r^2 >= 1e-15 for the cdist-based kernels, >= 1e-30 for the
 // Laplace double layers in FP64), self pairs (i == j inside a self segment)
-// contribute zero. The FP32 form clamps r^2 >= 1e-20 for every kernel.
+// contribute zero. The FP32 form uses r^2 + 1e-20 for every kernel (the bias is
+// below FP32 resolution for any r^2 > 1e-12 and keeps coincident points finite).
 
 template <typename R>
 struct Cplx {
@@ -62,9 +63,16 @@ struct Mth;
 
 template <>
 struct Mth<double> {
+  static constexpr bool fp32_bias = false;
   static constexpr double clamp_cd = 1e-15;
   static constexpr double clamp_dl = 1e-30;
-  __device__ static inline double rsq(double x) { return rsqrt(x); }
+  // 1/sqrt(x) for normal x > 0: hardware estimate (relative error ~5e-8) refined by one Newton
+  // step with its second-order term, y (1 + e/2 + 3e^2/8), e = 1 - x y^2 (measured ~1 ulp).
+  __device__ static inline double rsq(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(0.375, e, 0.5), y);
+  }
   __device__ static inline double lg(double x) { return log(x); }
   __device__ static inline void sc(double x, double* s, double* c) { sincos(x, s, c); }
   __device__ static inline double bj0(double x) { return j0(x); }
@@ -73,6 +81,7 @@ struct Mth<double> {
 
 template <>
 struct Mth<float> {
+  static constexpr bool fp32_bias = true;
   static constexpr float clamp_cd = 1e-20f;
   static constexpr float clamp_dl = 1e-20f;
   __device__ static inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -100,32 +109,41 @@ __device__ inline Cplx<R> kernel_value(R dx, R dy, R dz, R rn, R kappa) {
   using M = Mth<R>;
   constexpr R inv4pi = R(0.079577471545947667884441881686257181);
   Cplx<R> k;
-  R r2 = dx * dx + dy * dy;
-  if (OpT<OP>::dim == 3) r2 += dz * dz;
+  R r2;
+  if (M::fp32_bias) {
+    // FP32 form: r^2 is accumulated on top of the clamp (3 FMAs, no separate max)
+    r2 = fma(dx, dx, M::clamp_cd);
+    r2 = fma(dy, dy, r2);
+    if (OpT<OP>::dim == 3) r2 = fma(dz, dz, r2);
+  } else {
+    r2 = dx * dx + dy * dy;
+    if (OpT<OP>::dim == 3) r2 += dz * dz;
+  }
   if (OP == SKF_LAPLACE_2D) {
-    k.re = R(0.5) * M::lg(fmax(r2, M::clamp_cd));
+    k.re = R(0.5) * M::lg(M::fp32_bias ? r2 : fmax(r2, M::clamp_cd));
     k.im = 0;
   } else if (OP == SKF_LAPLACE_3D) {
-    k.re = M::rsq(fmax(r2, M::clamp_cd));
+    k.re = M::rsq(M::fp32_bias ? r2 : fmax(r2, M::clamp_cd));
     k.im = 0;
   } else if (OP == SKF_HELMHOLTZ_2D) {
-    R r = sqrt(fmax(r2, M::clamp_cd));
+    R r = sqrt(M::fp32_bias ? r2 : fmax(r2, M::clamp_cd));
     k.re = M::bj0(kappa * r);
     k.im = M::by0(kappa * r);
   } else if (OP == SKF_HELMHOLTZ_3D) {
-    R ir = M::rsq(fmax(r2, M::clamp_cd));
-    R r = fmax(r2, M::clamp_cd) * ir;
+    const R rc = M::fp32_bias ? r2 : fmax(r2, M::clamp_cd);
+    R ir = M::rsq(rc);
+    R r = rc * ir;
     R s, c;
     M::sc(kappa * r, &s, &c);
     k.re = c * ir;
     k.im = s * ir;
   } else if (OP == SKF_LAPLACE_DL_3D || OP == SKF_LAPLACE_ADL_3D) {
-    R ir = M::rsq(fmax(r2, M::clamp_dl));
+    R ir = M::rsq(M::fp32_bias ? r2 : fmax(r2, M::clamp_dl));
     R v = inv4pi * rn * ir * ir * ir;
     k.re = (OP == SKF_LAPLACE_DL_3D) ? -v : v;
     k.im = 0;
   } else {  // Helmholtz double layers: (1/4pi) rn e^{ikr} (ikr - 1) / r^3, sign flipped for the adjoint
-    R rc = fmax(r2, M::clamp_cd);
+    R rc = M::fp32_bias ? r2 : fmax(r2, M::clamp_cd);
     R ir = M::rsq(rc);
     R r = rc * ir;
     R s, c;
@@ -184,6 +202,42 @@ struct Stage {
   static constexpr int nv = (nraw + 1) & ~1;  // even -> 16-byte aligned rows
 };
 
+// Sum over `cnt` staged sources of one wave's tile for this lane's target. SELF skips the
+// source whose local index equals the target's (the self pair's zero diagonal).
+template <typename R, int OP, int NCH, bool SELF>
+__device__ inline void tile_sum(const R* __restrict__ tw, int cnt, int self_k, R xs0, R ys0, R zs0, R gxt, R gyt,
+                                R gzt, R kappa, R* part) {
+  using S = Stage<OP, NCH>;
+  constexpr int NV = S::nv;
+  constexpr int CW = S::cw;
+#pragma unroll 4
+  for (int k = 0; k < cnt; ++k) {
+    const R* row = tw + k * NV;
+    R dx = xs0 - row[0];
+    R dy = ys0 - row[1];
+    R dz = (S::dim == 3) ? zs0 - row[2] : R(0);
+    R rn = 0;
+    if (OpT<OP>::geom == 1) rn = dx * row[S::dim] + dy * row[S::dim + 1] + dz * row[S::dim + 2];
+    if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
+    Cplx<R> kv = kernel_value<R, OP>(dx, dy, dz, rn, kappa);
+    if (SELF && k == self_k) {
+      kv.re = 0;
+      kv.im = 0;
+    }
+    const R* qq = row + S::dim + S::ng;
+    if (CW == 1) {
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) part[c] += kv.re * qq[c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        part[2 * c] += kv.re * qq[2 * c] - kv.im * qq[2 * c + 1];
+        part[2 * c + 1] += kv.re * qq[2 * c + 1] + kv.im * qq[2 * c];
+      }
+    }
+  }
+}
+
 // One work item: up to 64 targets of one group against a range of its source entries.
 // The 4 waves of the workgroup split each 256-source tile (64 sources per wave);
 // partial sums are combined through LDS and added to the output with atomics.
@@ -200,7 +254,8 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
   const int ch0 = blockIdx.y * NCH;
   const int nchv = min(NCH, d.nch - ch0);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ng = d.g_n[g];
   const int nt = min(kTCH, ng - t0);
   const int tloc = t0 + lane;
@@ -267,38 +322,17 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
         }
       }
       __syncthreads();
-      const int wb = base + wave * 64;
-      const int cnt = max(0, min(64, ns - wb));
+      const int wb = base + wave * 64;  // wave-uniform
+      const int cnt = __builtin_amdgcn_readfirstlane(max(0, min(64, ns - wb)));
       R part[NA];
 #pragma unroll
       for (int a = 0; a < NA; ++a) part[a] = 0;
       const R* tw = &tile[wave * 64 * NV];
-#pragma unroll 4
-      for (int k = 0; k < cnt; ++k) {
-        const R* row = tw + k * NV;
-        R dx = xs0 - row[0];
-        R dy = ys0 - row[1];
-        R dz = (S::dim == 3) ? zs0 - row[2] : R(0);
-        R rn = 0;
-        if (OpT<OP>::geom == 1) rn = dx * row[S::dim] + dy * row[S::dim + 1] + dz * row[S::dim + 2];
-        if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
-        Cplx<R> kv = kernel_value<R, OP>(dx, dy, dz, rn, kappa);
-        if (self && (wb + k) == tloc) {
-          kv.re = 0;
-          kv.im = 0;
-        }
-        const R* qq = row + S::dim + S::ng;
-        if (CW == 1) {
-#pragma unroll
-          for (int c = 0; c < NCH; ++c) part[c] += kv.re * qq[c];
-        } else {
-#pragma unroll
-          for (int c = 0; c < NCH; ++c) {
-            part[2 * c] += kv.re * qq[2 * c] - kv.im * qq[2 * c + 1];
-            part[2 * c + 1] += kv.re * qq[2 * c + 1] + kv.im * qq[2 * c];
-          }
-        }
-      }
+      // the diagonal of a self segment falls in this tile only for targets tloc in [wb, wb + cnt)
+      if (self && tloc - wb >= 0 && tloc - wb < 64)
+        tile_sum<R, OP, NCH, true>(tw, cnt, tloc - wb, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part);
+      else
+        tile_sum<R, OP, NCH, false>(tw, cnt, -1, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part);
 #pragma unroll
       for (int a = 0; a < NA; ++a) acc[a] += sgn * double(part[a]);
       __syncthreads();
