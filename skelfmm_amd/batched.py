@@ -27,9 +27,9 @@ import numpy as np
 import torch
 
 from . import hip, operators as operators_mod
-from .layout import build_layout
+from .layout import build_layout, merge_layouts
 
-__all__ = ["SkelFMMBatched"]
+__all__ = ["SkelFMMBatched", "BatchedInstances"]
 
 
 class SkelFMMBatched(torch.nn.Module):
@@ -52,6 +52,19 @@ class SkelFMMBatched(torch.nn.Module):
 
     # ------------------------------------------------------------------ build
     def build_from_copy(self, skel_fmm, use_single_precision=False, device=None, max_sources=16384):
+        self.tol = skel_fmm.tol
+        self.root_level = skel_fmm.root_level
+        self.max_rank = skel_fmm.max_rank
+        self.max_bs = skel_fmm.max_bs
+        self.min_rank = skel_fmm.min_rank
+        self.max_rank_leaf = skel_fmm.max_rank_leaf
+        self.max_bs_leaf = skel_fmm.max_bs_leaf
+        self.rank_list = torch.as_tensor(skel_fmm.rank_list)
+        self.bs_list = torch.as_tensor(skel_fmm.bs_list)
+        return self.upload_layout(build_layout(skel_fmm, max_sources=max_sources), use_single_precision, device)
+
+    def upload_layout(self, L, use_single_precision=False, device=None):
+        """Copy a box layout to the GPU once; applies then reuse it."""
         hip.require_gpu()
         hip.lib()
         if device is None:
@@ -61,17 +74,7 @@ class SkelFMMBatched(torch.nn.Module):
         self.precision = 1 if use_single_precision else 0
         self.real_dtype = torch.float32 if use_single_precision else torch.float64
         self.complex_dtype = torch.complex64 if use_single_precision else torch.complex128
-        self.tol = skel_fmm.tol
-        self.root_level = skel_fmm.root_level
-        self.max_rank = skel_fmm.max_rank
-        self.max_bs = skel_fmm.max_bs
-        self.min_rank = skel_fmm.min_rank
-        self.max_rank_leaf = skel_fmm.max_rank_leaf
-        self.max_bs_leaf = skel_fmm.max_bs_leaf
-        self.rank_list_host = skel_fmm.rank_list.copy()
-        self.bs_list_host = skel_fmm.bs_list.copy()
-
-        L = build_layout(skel_fmm, max_sources=max_sources)
+        self.mfma_min_channels = 0
         self.layout = L
         dev = self.device
         rd = self.real_dtype
@@ -135,10 +138,10 @@ class SkelFMMBatched(torch.nn.Module):
         self._nf["items"] = i32(L.items.reshape(-1))
         self.interactions = L.interactions
         self.nslots = L.nslots
-        self.rank_list = torch.as_tensor(skel_fmm.rank_list)
-        self.bs_list = torch.as_tensor(skel_fmm.bs_list)
+        self.N = L.N
         for k in list(self._plans):
             self._destroy_plan(k)
+        self._graphs = {}
         return self
 
     def _p2p_args(self, width):
@@ -162,6 +165,7 @@ class SkelFMMBatched(torch.nn.Module):
             setattr(a, k, hip.ptr(nf[k]))
         a.nentries = int(nf["e_pt"].shape[0])
         a.nitems = int(nf["items"].shape[0] // 4)
+        a.mfma_min_channels = int(self.mfma_min_channels)
         return a
 
     def _plan(self, width):
@@ -237,6 +241,32 @@ class SkelFMMBatched(torch.nn.Module):
     def forward(self, q):
         return self.matvec(q)
 
+    def set_mfma_min_channels(self, n):
+        """Channel count from which FP32 real kernels use the matrix-core near field (0 default 16, <0 never)."""
+        self.mfma_min_channels = int(n)
+        for k in list(self._plans):
+            self._destroy_plan(k)
+        self._graphs = {}
+
+    # ------------------------------------------------------------ HIP graphs
+    def capture(self, x, out):
+        """Capture one apply ``x -> out`` (fixed device buffers) into a HIP graph and return it.
+
+        The apply issues only kernels (its zero fills are kernels, not memset nodes), so the
+        graph replays the whole apply with one launch call: ``graph.replay()``.
+        """
+        if x.shape[0] != self.N:
+            raise ValueError("charge vector length must match the number of points")
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            self.apply_device(x, out=out)  # creates the plan and its work vectors outside the capture
+        torch.cuda.current_stream(self.device).wait_stream(side)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.apply_device(x, out=out)
+        return g
+
     # ------------------------------------------------------------ accounting
     def nbytes_proj(self) -> int:
         return int(self.T.numel() * self.T.element_size())
@@ -250,3 +280,43 @@ class SkelFMMBatched(torch.nn.Module):
                 acc += t.numel() * t.element_size()
         acc += self.leaf_slot.numel() * 4 + self.leaf_user.numel() * 4
         return int(acc)
+
+
+class BatchedInstances(SkelFMMBatched):
+    """Independent point sets (same operator) applied together as one apply.
+
+    The builds' layouts are merged (``layout.merge_layouts``): one near-field launch
+    covers the interaction lists of every instance, and one upward and one downward
+    stage per tree level covers every instance that has that level. Charges and
+    results are the instances' arrays stacked along the point axis (``concat`` /
+    ``split``); right-hand sides ride along the second axis.
+    """
+
+    def __init__(self, builds, use_single_precision=False, device=None, max_sources=16384):
+        torch.nn.Module.__init__(self)
+        if not builds:
+            raise ValueError("need at least one build")
+        b0 = builds[0]
+        self.operator = operators_mod._require_operator(b0.operator)
+        if self.operator.kernel_id < 0:
+            raise NotImplementedError("the HIP apply supports the built-in operators of skelfmm_amd.operators only")
+        for b in builds:
+            if b.operator is not b0.operator or b.ndim != b0.ndim or float(b.kappa) != float(b0.kappa):
+                raise ValueError("all instances must share operator, dimension and kappa")
+        self.kappa = float(b0.kappa)
+        self.ndim = b0.ndim
+        self.kernel_backend = "hip"
+        self._plans = {}
+        self._graphs = {}
+        self.instance_sizes = [int(b.N) for b in builds]
+        self.ninstances = len(builds)
+        L = merge_layouts([build_layout(b, max_sources=max_sources) for b in builds])
+        self.upload_layout(L, use_single_precision, device)
+
+    def concat(self, xs):
+        """Stack per-instance charge arrays (N_i,) or (N_i, R) along the point axis on the device."""
+        return torch.cat([torch.as_tensor(x) for x in xs], dim=0).to(self.device)
+
+    def split(self, y):
+        """Per-instance views of a stacked result."""
+        return list(torch.split(y, self.instance_sizes, dim=0))

@@ -353,34 +353,214 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
   }
 }
 
-template <typename R, int OP, int NCH, bool PREC>
-void launch_p2p_t(const P2PDev& d, int nitems, hipStream_t st) {
-  dim3 grid(nitems, (d.nch + NCH - 1) / NCH);
-  hipLaunchKernelGGL((p2p_kernel<R, OP, NCH, PREC>), grid, dim3(kWG), 0, st, d);
+// Multi-RHS near field on the matrix cores (FP32 form, real kernels): for every 4 sources
+// a lane evaluates one kernel value K(t, s) with t = its wave's target (lane % 16) and
+// s = source (lane / 16) -- exactly the A-operand layout of v_mfma_f32_16x16x4_f32
+// (A[i][k] in lane i + 16k) -- and four MFMAs multiply it into 16x64 right-hand sides
+// read from LDS (B[k][j] in lane j + 16k). Wave w owns targets t0 + 16w .. t0 + 16w + 15;
+// D[4 (lane/16) + r][lane % 16] is accumulator r. FP32 MFMA sums of one 64-source tile are
+// added into FP64 accumulators after every tile.
+typedef float v4f __attribute__((ext_vector_type(4)));
+constexpr int kMB = 64;         // channels per launch block (4 MFMA column blocks of 16)
+constexpr int kQS = kMB + 16;   // padded LDS row: rows 4 apart land on distinct banks
+
+template <int OP, bool PREC>
+__global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
+  static_assert(OpT<OP>::cplx == 0, "real kernels only");
+  using S = Stage<OP, 1>;
+  constexpr int DIM = S::dim;
+  constexpr int NG = S::ng;
+  __shared__ float qt[64 * kQS];
+  __shared__ float pt[64 * 8];  // x, y, z, nx, ny, nz per source
+  const int4 it = d.items[blockIdx.x];
+  const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
+  const int c0 = blockIdx.y * kMB;
+  const int ncv = min(kMB, d.nch - c0);
+  const int nbv = (ncv + 15) / 16;  // valid 16-column blocks (uniform)
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ng = d.g_n[g];
+  const int tloc = t0 + wave * 16 + (lane & 15);  // target of this lane's A rows
+  const bool tvalid = tloc < ng;
+  const int tslot = d.g_pt[g] + (tvalid ? tloc : t0);
+  const float* px = static_cast<const float*>(d.px);
+  const float* py = static_cast<const float*>(d.py);
+  const float* pz = static_cast<const float*>(d.pz);
+  const float* qv = static_cast<const float*>(d.q);
+  float xt = px[tslot], yt = py[tslot], zt = (DIM == 3) ? pz[tslot] : 0.f;
+  float gxt = 0, gyt = 0, gzt = 0;
+  if (OpT<OP>::geom == 2) {
+    gxt = static_cast<const float*>(d.nx)[tslot];
+    gyt = static_cast<const float*>(d.ny)[tslot];
+    gzt = static_cast<const float*>(d.nz)[tslot];
+  }
+  double ctx = 0, cty = 0, ctz = 0;
+  if (PREC) {
+    const int gb = d.g_box[g];
+    ctx = d.centers[3 * gb + 0];
+    cty = d.centers[3 * gb + 1];
+    ctz = d.centers[3 * gb + 2];
+  }
+  double acc[4][4];
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[nb][r] = 0.0;
+
+  for (int s = sb; s < se; ++s) {
+    const int ps = d.e_pt[s], ns = d.e_n[s], qo = d.e_q[s], fl = d.e_flags[s];
+    const bool self = (fl & SKF_SRC_SELF) != 0;
+    const double sgn = (fl & SKF_SRC_NEG) ? -1.0 : 1.0;
+    float xs0 = xt, ys0 = yt, zs0 = zt;
+    if (PREC) {
+      const int eb = d.e_box[s];
+      xs0 = xt + float(ctx - d.centers[3 * eb + 0]);
+      ys0 = yt + float(cty - d.centers[3 * eb + 1]);
+      zs0 = zt + float(ctz - d.centers[3 * eb + 2]);
+    }
+    for (int base = 0; base < ns; base += 64) {
+      const int nsrc = min(64, ns - base);
+      // stage 64 sources: coordinates (threads 0..63) and the 64 x 64 charge block (all threads)
+      if (tid < 64) {
+        const int j = base + tid;
+        float* row = &pt[tid * 8];
+        if (tid < nsrc) {
+          const int sl = ps + j;
+          row[0] = px[sl];
+          row[1] = py[sl];
+          row[2] = (DIM == 3) ? pz[sl] : 0.f;
+          if (NG) {
+            row[3] = static_cast<const float*>(d.nx)[sl];
+            row[4] = static_cast<const float*>(d.ny)[sl];
+            row[5] = static_cast<const float*>(d.nz)[sl];
+          }
+        } else {
+          // far away, zero charge: contributes exactly zero
+          row[0] = 1e18f;
+          row[1] = 1e18f;
+          row[2] = 1e18f;
+          row[3] = row[4] = row[5] = 0.f;
+        }
+      }
+      {
+        const int sr = tid >> 2, cq = (tid & 3) * 16;
+        float* dst = &qt[sr * kQS + cq];
+        if (sr < nsrc) {
+          const float* src = qv + (long long)(qo + base + sr) * d.width + c0 + cq;
+#pragma unroll
+          for (int c = 0; c < 16; ++c) dst[c] = (cq + c < ncv) ? src[c] : 0.f;
+        } else {
+#pragma unroll
+          for (int c = 0; c < 16; ++c) dst[c] = 0.f;
+        }
+      }
+      __syncthreads();
+      v4f D[4];
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) D[nb] = v4f{0.f, 0.f, 0.f, 0.f};
+      const int ksteps = (nsrc + 3) >> 2;
+      for (int kk = 0; kk < ksteps; ++kk) {
+        const int sj = kk * 4 + (lane >> 4);
+        const float* row = &pt[sj * 8];
+        const float dx = xs0 - row[0], dy = ys0 - row[1], dz = (DIM == 3) ? zs0 - row[2] : 0.f;
+        float rn = 0.f;
+        if (OpT<OP>::geom == 1) rn = dx * row[3] + dy * row[4] + dz * row[5];
+        if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
+        float a = kernel_value<float, OP>(dx, dy, dz, rn, 0.f).re;
+        if (self && (base + sj) == tloc) a = 0.f;
+        const float* qrow = &qt[sj * kQS + (lane & 15)];
+#pragma unroll
+        for (int nb = 0; nb < 4; ++nb)
+          if (nb < nbv) D[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, qrow[nb * 16], D[nb], 0, 0, 0);
+      }
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[nb][r] += sgn * double(D[nb][r]);
+      __syncthreads();
+    }
+  }
+  // D[4 (lane/16) + r][lane % 16] of column block nb -> target t0 + 16 wave + 4 (lane/16) + r, channel 16 nb + lane % 16
+  float* uo = static_cast<float*>(d.u);
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) {
+    const int ch = nb * 16 + (lane & 15);
+    if (ch >= ncv) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int tl = t0 + wave * 16 + 4 * (lane >> 4) + r;
+      if (tl < ng) atomic_acc(&uo[(long long)(d.g_out[g] + tl) * d.width + c0 + ch], float(acc[nb][r]));
+    }
+  }
 }
 
-template <typename R, bool PREC, int NCH>
-int launch_p2p_op(int op, const P2PDev& d, int nitems, hipStream_t st) {
-  switch (op) {
-    case SKF_LAPLACE_2D: launch_p2p_t<R, SKF_LAPLACE_2D, NCH, PREC>(d, nitems, st); break;
-    case SKF_LAPLACE_3D: launch_p2p_t<R, SKF_LAPLACE_3D, NCH, PREC>(d, nitems, st); break;
-    case SKF_HELMHOLTZ_2D: launch_p2p_t<R, SKF_HELMHOLTZ_2D, NCH, PREC>(d, nitems, st); break;
-    case SKF_HELMHOLTZ_3D: launch_p2p_t<R, SKF_HELMHOLTZ_3D, NCH, PREC>(d, nitems, st); break;
-    case SKF_LAPLACE_DL_3D: launch_p2p_t<R, SKF_LAPLACE_DL_3D, NCH, PREC>(d, nitems, st); break;
-    case SKF_LAPLACE_ADL_3D: launch_p2p_t<R, SKF_LAPLACE_ADL_3D, NCH, PREC>(d, nitems, st); break;
-    case SKF_HELMHOLTZ_DL_3D: launch_p2p_t<R, SKF_HELMHOLTZ_DL_3D, NCH, PREC>(d, nitems, st); break;
-    case SKF_HELMHOLTZ_ADL_3D: launch_p2p_t<R, SKF_HELMHOLTZ_ADL_3D, NCH, PREC>(d, nitems, st); break;
-    default: return set_err(SKF_ERR_ARG, "unknown kernel id %d", op);
-  }
+template <int OP>
+int launch_p2p_mfma(const P2PDev& d, int nitems, hipStream_t st) {
+  dim3 grid(nitems, (d.nch + kMB - 1) / kMB);
+  hipLaunchKernelGGL((p2p_mfma_kernel<OP, true>), grid, dim3(kWG), 0, st, d);
   SKF_LAUNCHED();
   return SKF_OK;
 }
 
+// static LDS of one p2p_kernel instance (tile + cross-wave reduction); kept within 64 KiB
+template <typename R, int OP, int NCH>
+constexpr int p2p_lds_bytes() {
+  using S = Stage<OP, NCH>;
+  return kWG * S::nv * int(sizeof(R)) + 4 * kTCH * NCH * S::cw * 8;
+}
+
+template <typename R, int OP, int NCH, bool PREC>
+int launch_p2p_t(const P2PDev& d, int nitems, hipStream_t st) {
+  if constexpr (p2p_lds_bytes<R, OP, NCH>() <= 65536) {
+    dim3 grid(nitems, (d.nch + NCH - 1) / NCH);
+    hipLaunchKernelGGL((p2p_kernel<R, OP, NCH, PREC>), grid, dim3(kWG), 0, st, d);
+    return SKF_OK;
+  } else {
+    return set_err(SKF_ERR_UNSUPPORTED, "channel block too large for LDS");
+  }
+}
+
+template <typename R, bool PREC, int NCH>
+int launch_p2p_op(int op, const P2PDev& d, int nitems, hipStream_t st) {
+  int rc;
+  switch (op) {
+    case SKF_LAPLACE_2D: rc = launch_p2p_t<R, SKF_LAPLACE_2D, NCH, PREC>(d, nitems, st); break;
+    case SKF_LAPLACE_3D: rc = launch_p2p_t<R, SKF_LAPLACE_3D, NCH, PREC>(d, nitems, st); break;
+    case SKF_HELMHOLTZ_2D: rc = launch_p2p_t<R, SKF_HELMHOLTZ_2D, NCH, PREC>(d, nitems, st); break;
+    case SKF_HELMHOLTZ_3D: rc = launch_p2p_t<R, SKF_HELMHOLTZ_3D, NCH, PREC>(d, nitems, st); break;
+    case SKF_LAPLACE_DL_3D: rc = launch_p2p_t<R, SKF_LAPLACE_DL_3D, NCH, PREC>(d, nitems, st); break;
+    case SKF_LAPLACE_ADL_3D: rc = launch_p2p_t<R, SKF_LAPLACE_ADL_3D, NCH, PREC>(d, nitems, st); break;
+    case SKF_HELMHOLTZ_DL_3D: rc = launch_p2p_t<R, SKF_HELMHOLTZ_DL_3D, NCH, PREC>(d, nitems, st); break;
+    case SKF_HELMHOLTZ_ADL_3D: rc = launch_p2p_t<R, SKF_HELMHOLTZ_ADL_3D, NCH, PREC>(d, nitems, st); break;
+    default: return set_err(SKF_ERR_ARG, "unknown kernel id %d", op);
+  }
+  if (rc) return rc;
+  SKF_LAUNCHED();
+  return SKF_OK;
+}
+
+// channels evaluated per kernel value: the widest block whose LDS fits (FP64: 8 real / 4 complex,
+// FP32: 16 real / 8 complex); multi-RHS FP32 real kernels go to the matrix-core kernel instead
 template <typename R, bool PREC>
-int launch_p2p_prec(int op, const P2PDev& d, int nitems, hipStream_t st) {
+int launch_p2p_prec(int op, const P2PDev& d, int nitems, hipStream_t st, int mfma_min) {
+  const bool cx = (op == SKF_HELMHOLTZ_2D || op == SKF_HELMHOLTZ_3D || op == SKF_HELMHOLTZ_DL_3D ||
+                   op == SKF_HELMHOLTZ_ADL_3D);
+  if constexpr (sizeof(R) == 4 && PREC) {
+    if (!cx && mfma_min > 0 && d.nch >= mfma_min) {
+      switch (op) {
+        case SKF_LAPLACE_2D: return launch_p2p_mfma<SKF_LAPLACE_2D>(d, nitems, st);
+        case SKF_LAPLACE_3D: return launch_p2p_mfma<SKF_LAPLACE_3D>(d, nitems, st);
+        case SKF_LAPLACE_DL_3D: return launch_p2p_mfma<SKF_LAPLACE_DL_3D>(d, nitems, st);
+        case SKF_LAPLACE_ADL_3D: return launch_p2p_mfma<SKF_LAPLACE_ADL_3D>(d, nitems, st);
+        default: break;
+      }
+    }
+  }
   if (d.nch == 1) return launch_p2p_op<R, PREC, 1>(op, d, nitems, st);
   if (d.nch <= 2) return launch_p2p_op<R, PREC, 2>(op, d, nitems, st);
-  return launch_p2p_op<R, PREC, 4>(op, d, nitems, st);
+  if (d.nch <= 4 || (sizeof(R) == 8 && cx)) return launch_p2p_op<R, PREC, 4>(op, d, nitems, st);
+  if (d.nch <= 8 || sizeof(R) == 8 || cx) return launch_p2p_op<R, PREC, 8>(op, d, nitems, st);
+  return launch_p2p_op<R, PREC, 16>(op, d, nitems, st);
 }
 
 bool op_is_complex(int op) {
@@ -409,8 +589,9 @@ int run_p2p(const skf_p2p_args* a, const void* q, void* u, hipStream_t st) {
   d.width = a->width;
   d.nch = cx ? a->width / 2 : a->width;
   d.kappa = a->kappa;
-  if (a->precision == 0) return launch_p2p_prec<double, false>(a->kernel, d, a->nitems, st);
-  if (a->precision == 1) return launch_p2p_prec<float, true>(a->kernel, d, a->nitems, st);
+  const int mfma_min = a->mfma_min_channels == 0 ? 16 : a->mfma_min_channels;
+  if (a->precision == 0) return launch_p2p_prec<double, false>(a->kernel, d, a->nitems, st, mfma_min);
+  if (a->precision == 1) return launch_p2p_prec<float, true>(a->kernel, d, a->nitems, st, mfma_min);
   return set_err(SKF_ERR_ARG, "precision must be 0 or 1");
 }
 

@@ -44,7 +44,7 @@ class P2PArgs(ctypes.Structure):
         ("centers", _vp), ("q", _vp), ("u", _vp),
         ("ngroups", _i), ("g_pt", _vp), ("g_n", _vp), ("g_out", _vp), ("g_box", _vp),
         ("nentries", _i), ("e_pt", _vp), ("e_n", _vp), ("e_q", _vp), ("e_box", _vp), ("e_flags", _vp),
-        ("nitems", _i), ("items", _vp),
+        ("nitems", _i), ("items", _vp), ("mfma_min_channels", _i),
     ]
 
 

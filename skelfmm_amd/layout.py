@@ -170,6 +170,102 @@ def build_layout(fmm, max_sources=16384):
     return L
 
 
+def merge_layouts(layouts):
+    """One layout for several independent instances (same operator and dimension).
+
+    Slots, boxes, points, T and work lists are concatenated with offsets; the level
+    stages of all instances that sit on the same tree level are merged into one stage,
+    so one upward and one downward launch per level serve every instance (each
+    instance still runs its levels deepest first upward and shallowest first downward).
+    """
+    if len(layouts) == 1:
+        return layouts[0]
+    M = BoxLayout()
+    M.ndim = layouts[0].ndim
+    M.N = sum(L.N for L in layouts)
+    M.nboxes = sum(L.nboxes for L in layouts)
+    M.nslots = sum(L.nslots for L in layouts)
+    slot_base = np.cumsum([0] + [L.nslots for L in layouts])[:-1]
+    box_base = np.cumsum([0] + [L.nboxes for L in layouts])[:-1]
+    pt_base = np.cumsum([0] + [L.N for L in layouts])[:-1]
+    t_base = np.cumsum([0] + [L.T.shape[0] for L in layouts])[:-1]
+    M.instance_points = np.array([L.N for L in layouts], dtype=np.int64)
+    M.instance_point_base = pt_base.astype(np.int64)
+    M.off = np.concatenate([np.where(L.off >= 0, L.off + sb, -1) for L, sb in zip(layouts, slot_base)])
+    M.slot_point = np.concatenate([L.slot_point + pb for L, pb in zip(layouts, pt_base)])
+    M.slot_box = np.concatenate([L.slot_box + bb for L, bb in zip(layouts, box_base)])
+    M.points = np.concatenate([L.points for L in layouts], axis=1)
+    M.normals = None if layouts[0].normals is None else np.concatenate([L.normals for L in layouts], axis=1)
+    M.centers = np.concatenate([L.centers for L in layouts])
+    M.leaf_slot = np.concatenate([L.leaf_slot + sb for L, sb in zip(layouts, slot_base)]).astype(np.int32)
+    M.leaf_user = np.concatenate([L.leaf_user + pb for L, pb in zip(layouts, pt_base)]).astype(np.int32)
+    M.T = np.concatenate([L.T for L in layouts])
+    M.toff = np.concatenate([L.toff + tb for L, tb in zip(layouts, t_base)])
+    M.root_level = min(L.root_level for L in layouts)
+
+    def merge_stages(stage_lists, descending):
+        by_level = {}
+        for i, stages in enumerate(stage_lists):
+            for S in stages:
+                by_level.setdefault(S.level, []).append((i, S))
+        out = []
+        for lev in sorted(by_level, reverse=descending):
+            parts = by_level[lev]
+            boxes, off, bsz, rank, toff, gd, gs, items = [], [], [], [], [], [], [], []
+            nbox = 0
+            for i, S in parts:
+                boxes.append(S.boxes + box_base[i])
+                off.append(S.box_off + slot_base[i])
+                bsz.append(S.box_bs)
+                rank.append(S.box_rank)
+                toff.append(S.box_toff + t_base[i])
+                gd.append(S.gather_dst + slot_base[i])
+                gs.append(S.gather_src + slot_base[i])
+                it = S.items.copy()
+                it[:, 0] += nbox
+                items.append(it)
+                nbox += S.box_off.shape[0]
+            out.append(LevelStage(lev, np.concatenate(boxes), np.concatenate(off).astype(np.int32),
+                                  np.concatenate(bsz).astype(np.int32), np.concatenate(rank).astype(np.int32),
+                                  np.concatenate(toff).astype(np.int64), np.concatenate(gd).astype(np.int32),
+                                  np.concatenate(gs).astype(np.int32),
+                                  np.concatenate(items).astype(np.int32).reshape(-1, 2)))
+        return out
+
+    M.up = merge_stages([L.up for L in layouts], True)
+    M.down = merge_stages([L.down for L in layouts], False)
+
+    g_base = np.cumsum([0] + [L.g_pt.shape[0] for L in layouts])[:-1]
+    e_base = np.cumsum([0] + [L.e_pt.shape[0] for L in layouts])[:-1]
+    M.g_pt = np.concatenate([L.g_pt + sb for L, sb in zip(layouts, slot_base)]).astype(np.int32)
+    M.g_n = np.concatenate([L.g_n for L in layouts]).astype(np.int32)
+    M.g_kind = np.concatenate([L.g_kind for L in layouts]).astype(np.int32)
+    M.g_out = (M.g_pt + M.g_kind * M.nslots).astype(np.int32)
+    M.g_box = np.concatenate([L.g_box + bb for L, bb in zip(layouts, box_base)]).astype(np.int32)
+    M.e_pt = np.concatenate([L.e_pt + sb for L, sb in zip(layouts, slot_base)]).astype(np.int32)
+    M.e_n = np.concatenate([L.e_n for L in layouts]).astype(np.int32)
+    M.e_qkind = np.concatenate([L.e_qkind for L in layouts]).astype(np.int32)
+    M.e_q = (M.e_pt + M.e_qkind * M.nslots).astype(np.int32)
+    M.e_box = np.concatenate([L.e_box + bb for L, bb in zip(layouts, box_base)]).astype(np.int32)
+    M.e_flags = np.concatenate([L.e_flags for L in layouts]).astype(np.int32)
+    items = []
+    costs = []
+    for L, gb, eb in zip(layouts, g_base, e_base):
+        it = L.items.copy()
+        it[:, 0] += gb
+        it[:, 2] += eb
+        it[:, 3] += eb
+        items.append(it)
+        costs.append(L.item_cost)
+    items = np.concatenate(items)
+    costs = np.concatenate(costs)
+    order = np.argsort(-costs, kind="stable")
+    M.items = np.ascontiguousarray(items[order]).astype(np.int32)
+    M.item_cost = costs[order]
+    M.interactions = int(sum(L.interactions for L in layouts))
+    return M
+
+
 def _near_field(L, fmm, off, bs, rk, max_sources):
     nslots = L.nslots
     # (target box, target kind) -> list of (source box, source kind, charge kind, flags); kind 0 = all, 1 = skel
@@ -251,6 +347,8 @@ def _near_field(L, fmm, off, bs, rk, max_sources):
     L.g_n = g_n.astype(np.int32)
     L.g_out = g_out.astype(np.int32)
     L.g_box = g_box.astype(np.int32)
+    L.g_kind = g_kind.astype(np.int32)
+    L.e_qkind = R[:, 4].astype(np.int32)
     L.e_pt = e_pt.astype(np.int32)
     L.e_n = e_n.astype(np.int32)
     L.e_q = e_q.astype(np.int32)

@@ -227,3 +227,110 @@ def test_skelfmm_public_api(capsys):
     assert single.apply(x).dtype == np.float32
     hz = SkelFMM(X[:1500], ops.HELMHOLTZ_2D, tol=1e-6, leaf_size=64, param=4.0)
     assert hz.apply(x[:1500]).dtype == np.complex128
+
+
+def _annulus_instances(ns, seed=2):
+    from conftest import annulus_points
+
+    return [annulus_points(n, seed + i) for i, n in enumerate(ns)]
+
+
+def test_batched_instances_match_separate_applies():
+    from skelfmm_amd.batched import BatchedInstances
+
+    pts = _annulus_instances((3000, 9000, 1200, 5000))
+    builds = [make_build(P, ops.LAPLACE_2D, tol=1e-5, leaf_size=64) for P in pts]
+    rng = np.random.default_rng(0)
+    qs = [rng.normal(size=(P.shape[0], 3)) for P in pts]
+    dev = BatchedInstances(builds)
+    y = dev.matvec(dev.concat(qs))
+    torch.cuda.synchronize()
+    for part, b, q in zip(dev.split(y), builds, qs):
+        ref = BatchedApplyCPU(b, bytes_budget=2e8).matvec(q).numpy()
+        assert rel_l2(part.cpu().numpy(), ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("name,nrhs", [("laplace_2d", 64), ("laplace_3d", 20), ("laplace_dl", 16)])
+def test_mfma_multi_rhs_precision_form(name, nrhs):
+    """FP32 matrix-core near field (R >= 16) against the FP64 apply on the same build."""
+    from skelfmm_amd.batched import SkelFMMBatched
+
+    rng = np.random.default_rng(1)
+    if name == "laplace_2d":
+        X, op, g, leaf = annulus_points(6000, 5), ops.LAPLACE_2D, None, 64
+    elif name == "laplace_3d":
+        X, op, g, leaf = rng.random((12000, 3)), ops.LAPLACE_3D, None, 150
+    else:
+        # volume cloud with random unit normals (a surface with normals = positions is ill-conditioned
+        # in FP32 coordinates; see test_fp32_double_layer_surface_conditioning)
+        X = rng.random((12000, 3))
+        g = rng.normal(size=(12000, 3))
+        g /= np.linalg.norm(g, axis=1, keepdims=True)
+        op, leaf = ops.LAPLACE_DOUBLE_LAYER_3D, 150
+    fmm = make_build(X, op, tol=1e-6, leaf_size=leaf, geometry=g)
+    Q = rng.normal(size=(X.shape[0], nrhs))
+    d64 = SkelFMMBatched(fmm.tree, op).build_from_copy(fmm)
+    y64 = d64.matvec(torch.as_tensor(Q, device=_gpu())).cpu().numpy()
+    d32 = SkelFMMBatched(fmm.tree, op).build_from_copy(fmm, use_single_precision=True)
+    q32 = torch.as_tensor(Q.astype(np.float32), device=_gpu())
+    y_mfma = d32.matvec(q32).cpu().numpy().astype(np.float64)
+    d32.set_mfma_min_channels(-1)
+    y_valu = d32.matvec(q32).cpu().numpy().astype(np.float64)
+    e_mfma = rel_l2(y_mfma, y64)
+    e_valu = rel_l2(y_valu, y64)
+    print(f"{name} R={nrhs}: MFMA {e_mfma:.2e} VALU {e_valu:.2e}", flush=True)
+    bound = 2e-5 if name == "laplace_dl" else 2e-6  # 1/r^2 double-layer sums cancel more than 1/r sums
+    assert e_mfma < bound and e_valu < bound
+    assert rel_l2(y_mfma, y_valu) < 1e-6
+    assert rel_l2(y64, BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(Q).numpy()) <= FP64_TOL
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_graph_replay_matches_eager(single):
+    from skelfmm_amd.batched import SkelFMMBatched
+
+    X = np.random.default_rng(3).random((20000, 3))
+    fmm = make_build(X, ops.LAPLACE_3D, tol=1e-6, leaf_size=150)
+    dev = SkelFMMBatched(fmm.tree, ops.LAPLACE_3D).build_from_copy(fmm, use_single_precision=single)
+    dt = torch.float32 if single else torch.float64
+    x = torch.randn(X.shape[0], 2, dtype=dt, device=_gpu())
+    out = torch.empty_like(x)
+    eager = dev.apply_device(x).clone()
+    g = dev.capture(x, out)
+    for rep in range(4):
+        out.zero_()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, eager) or rel_l2(out.cpu().numpy(), eager.cpu().numpy()) < (1e-6 if single else 1e-14)
+        # eager work between replays
+        tmp = dev.apply_device(torch.randn_like(x))
+        torch.cuda.synchronize()
+        del tmp
+    x.copy_(x * 2.0)
+    g.replay()
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), 2.0 * eager.cpu().numpy()) < (1e-6 if single else 1e-14)
+
+
+def test_fp32_double_layer_surface_conditioning():
+    """Double layer on the unit sphere with normals = positions: rn = (x - y).n = -r^2/2 is
+    ill-conditioned in the coordinates (rounding only the points to FP32 costs ~4e-3 in an exact
+    sum). The box-centred FP32 form must do at least as well as the oracle's restatement of the
+    reference's single-precision form on the same build, with FP32 matrix-core and VALU paths equal."""
+    from skelfmm_amd.batched import SkelFMMBatched
+
+    X = sphere_points(5000, 4)
+    fmm = make_build(X, ops.LAPLACE_DOUBLE_LAYER_3D, tol=1e-6, leaf_size=100, geometry=X)
+    Q = np.random.default_rng(1).normal(size=(X.shape[0], 16))
+    y64 = BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(Q).numpy()
+    ref32 = BatchedApplyCPU(fmm, use_single_precision=True, bytes_budget=2e8).matvec(Q.astype(np.float32)).numpy()
+    d32 = SkelFMMBatched(fmm.tree, fmm.operator).build_from_copy(fmm, use_single_precision=True)
+    q32 = torch.as_tensor(Q.astype(np.float32), device=_gpu())
+    y_mfma = d32.matvec(q32).cpu().numpy().astype(np.float64)
+    d32.set_mfma_min_channels(-1)
+    y_valu = d32.matvec(q32).cpu().numpy().astype(np.float64)
+    e_ref = rel_l2(ref32.astype(np.float64), y64)
+    e_gpu = rel_l2(y_mfma, y64)
+    print(f"sphere DL FP32: oracle single-precision form {e_ref:.2e}, GPU precision form {e_gpu:.2e}", flush=True)
+    assert e_gpu <= e_ref
+    assert rel_l2(y_mfma, y_valu) < 1e-5

@@ -123,3 +123,23 @@ def test_interaction_count_matches_lists():
     lst = f.uskel_from_q_list
     n += 2 * int(np.sum(rk[lst[:, 0]] * bs[lst[:, 1]])) if lst.shape[0] else 0
     assert L.interactions == n
+
+
+def test_merged_layout_emulates_every_instance():
+    """Three independent instances in one layout reproduce their separate oracle applies."""
+    from skelfmm_amd.layout import merge_layouts
+
+    rng = np.random.default_rng(8)
+    builds = [make_build(annulus_points(n, s), ops.LAPLACE_2D, tol=1e-5, leaf_size=40)
+              for n, s in ((1500, 1), (4000, 2), (900, 3))]
+    L = merge_layouts([build_layout(b, max_sources=600) for b in builds])
+    assert L.N == sum(b.N for b in builds)
+    qs = [rng.normal(size=b.N) for b in builds]
+    y = emulate_apply(L, "laplace_2d", 0.0, np.concatenate(qs))
+    off = 0
+    for b, q in zip(builds, qs):
+        ref = BatchedApplyCPU(b, bytes_budget=1e8).matvec(q).numpy()
+        assert rel_l2(y[off:off + b.N], ref) < 1e-13
+        off += b.N
+    levels = [S.level for S in L.up]
+    assert levels == sorted(set(levels), reverse=True)
