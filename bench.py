@@ -15,7 +15,12 @@ duration, timed with HIP events on the stream it runs on.
 sampled fraction of every interaction list plus full interpolation passes,
 extrapolated by interaction count.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu]
+Second workload (``--workload annuli``): 32 independent curvy annuli (2D Laplace,
+N_i log-uniform in [1e4, 5e5], tol 1e-5, leaf 128) with 64 float32 right-hand
+sides each, applied together as one graph-captured apply.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-tune]
+       python bench.py --workload annuli
 """
 
 from __future__ import annotations
@@ -44,6 +49,144 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def annuli_instances(n_inst=32, nrhs=64, seed=2):
+    """32 curvy annuli with N_i log-uniform in [1e4, 5e5] and float32 N(0,1) charges (N_i, nrhs).
+
+    rng = default_rng(seed): N_i = round(exp(U(ln 1e4, ln 5e5))) for all sets first, then per set
+    theta ~ U[0, 2pi), u ~ U[0, 1), r = (1 + 0.25 cos 6 theta)(0.5 + 0.5 u), then the charges.
+    """
+    rng = np.random.default_rng(seed)
+    ns = np.round(np.exp(rng.uniform(np.log(1e4), np.log(5e5), n_inst))).astype(np.int64)
+    pts, qs = [], []
+    for n in ns:
+        th = rng.uniform(0.0, 2 * np.pi, n)
+        u = rng.uniform(0.0, 1.0, n)
+        r = (1.0 + 0.25 * np.cos(6 * th)) * (0.5 + 0.5 * u)
+        pts.append(np.stack([r * np.cos(th), r * np.sin(th)], axis=1))
+        qs.append(rng.standard_normal((n, nrhs), dtype=np.float32))
+    return ns, pts, qs
+
+
+def run_annuli(args):
+    """Batched uneven 2D Laplace: every instance and RHS in one graph-captured apply."""
+    import torch
+
+    from oracle.direct import direct_sum_c
+    from skelfmm_amd import autotune, hip, operators
+    from skelfmm_amd.batched import BatchedInstances
+    from skelfmm_amd.build import RecursiveSkeletonization
+    from skelfmm_amd.tree import BalancedTree
+
+    hip.require_gpu()
+    dev = torch.device("cuda", 0)
+    op = operators.LAPLACE_2D
+    tol, leaf = 1e-5, 128
+    ns, pts, qs = annuli_instances(args.instances, args.nrhs)
+    t0 = time.perf_counter()
+    builds = []
+    for P in pts:
+        tree = BalancedTree(P, leaf_size=leaf)
+        f = RecursiveSkeletonization(tree, op, kappa=0.0, tol=tol)
+        f.skel_tree(npoints_max=500)
+        f.setup_lists()
+        builds.append(f)
+    t_host = time.perf_counter() - t0
+    d32 = BatchedInstances(builds, use_single_precision=True)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    d64 = BatchedInstances(builds, use_single_precision=False)
+    ntot = int(ns.sum())
+    log(f"annuli: {len(ns)} instances, {ntot} points, {args.nrhs} RHS; build {build_s:.1f}s (host {t_host:.1f}); "
+        f"interactions/apply {d32.interactions:.3e}")
+    width = args.nrhs
+    tuned = None
+    if not args.no_tune:
+        tuned = autotune.tune(d32, width)
+        log(f"autotune FP32: {tuned}")
+    x32 = d32.concat([torch.as_tensor(q) for q in qs])
+    x64 = x32.to(torch.float64)
+
+    def graph_time(d, x):
+        out = torch.empty_like(x)
+        g = d.capture(x, out)
+        for _ in range(args.warmup):
+            g.replay()
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - ts) / args.steps, out
+
+    t32, y32 = graph_time(d32, x32)
+    t64, y64 = graph_time(d64, x64)
+    p32 = d32.time_near_field(width, reps=5) * 1e-3
+    log(f"apply (graph replay) FP32 {t32 * 1e3:.2f} ms, FP64 {t64 * 1e3:.2f} ms; near field FP32 {p32 * 1e3:.2f} ms")
+
+    # per-instance relative l2 error on 4 RHS columns at 100 sampled targets vs the FP64 direct sum
+    errs32, errs64 = [], []
+    parts32 = d32.split(y32)
+    parts64 = d64.split(y64)
+    for i, (P, q) in enumerate(zip(pts, qs)):
+        idx = np.random.default_rng(i).choice(P.shape[0], min(100, P.shape[0]), replace=False)
+        ref = direct_sum_c(op.name, P[idx], P, q[:, :4].astype(np.float64))
+        a = parts32[i][idx, :4].cpu().numpy().astype(np.float64)
+        b = parts64[i][idx, :4].cpu().numpy()
+        errs32.append(float(np.linalg.norm(a - ref) / np.linalg.norm(ref)))
+        errs64.append(float(np.linalg.norm(b - ref) / np.linalg.norm(ref)))
+    log(f"rel-l2 vs direct sum (max over instances): FP32 {max(errs32):.3e}, FP64 {max(errs64):.3e}")
+
+    mfma_tf = 2.0 * d32.interactions * width / p32 / 1e12
+    cpu = None
+    if not args.no_cpu:
+        from oracle.batched_apply import BatchedApplyCPU
+
+        ncores = min(16, len(os.sched_getaffinity(0)))
+        torch.set_num_threads(ncores)
+        k = int(np.argmin(ns))
+        ba = BatchedApplyCPU(builds[k], bytes_budget=10e9, pair_fraction=0.25)
+        q0 = qs[k].astype(np.float64)
+        ba.matvec(q0)
+        tc = time.perf_counter()
+        ba.matvec(q0)
+        el = time.perf_counter() - tc
+        est = el * d64.interactions / max(ba.interactions, 1)
+        cpu = {"value": round(ntot * width / est / 1e6, 6), "unit": "Mpoints*RHS/s", "cores": ncores,
+               "kind": "port",
+               "sample": (f"padded torch-CPU FP64 restatement of the reference apply, {width} RHS, on 25% of the pairs "
+                          f"of the smallest instance (N={int(ns[k])}, {ba.interactions:.3e} interactions, {el:.1f}s), "
+                          f"extrapolated to {d64.interactions:.3e} interactions")}
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": "Mpoints*RHS/s per apply, FP32 precision form (FP64 form beside it)",
+        "value": round(ntot * width / t32 / 1e6, 3),
+        "unit": "Mpoints*RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t32 * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: 32 curvy annuli, N_i log-uniform in [1e4, 5e5] (default_rng(2)), float32 N(0,1) charges",
+        "config": {"workload": "batched_annuli_laplace_2d", "instances": int(len(ns)), "N_total": ntot,
+                   "nrhs": width, "tol": tol, "leaf_size": leaf, "graph": True,
+                   "interactions_per_apply": int(d32.interactions), "autotune": tuned},
+        "fp64": {"value": round(ntot * width / t64 / 1e6, 3), "ms_per_step": round(t64 * 1e3, 4),
+                 "max_rel_l2_err": max(errs64)},
+        "fp32_max_rel_l2_err": max(errs32),
+        "build_s": round(build_s, 2),
+        "roofline": {"bound": "mfma", "achieved": round(mfma_tf, 3), "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                     "frac": round(mfma_tf / PEAK_TFLOPS["fp32"], 4), "traffic": None,
+                     "kernel": "near field (matrix-core kernel at >= 16 RHS unless the autotuner picked VALU)",
+                     "kernel_ms": round(p32 * 1e3, 4), "flops_counted": "2 x interactions x RHS (K q products)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -57,11 +200,18 @@ def main():
     ap.add_argument("--ncheck", type=int, default=1000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 512 MiB device-to-device copy first (calibrates rocprofv3 byte counters)")
+    ap.add_argument("--workload", choices=("laplace3d_cube", "annuli"), default="laplace3d_cube",
+                    help="laplace3d_cube: 1M-point 3D Laplace (default); annuli: 32 batched 2D annuli x 64 RHS")
+    ap.add_argument("--instances", type=int, default=32)
+    ap.add_argument("--nrhs", type=int, default=64)
+    ap.add_argument("--no-tune", action="store_true", help="skip the near-field autotuner (on by default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "annuli":
+        return run_annuli(args)
 
     import torch
 
@@ -107,6 +257,12 @@ def main():
         f"levels {tree.nlevels}, root {fmm.root_level}, max rank {fmm.max_rank}, "
         f"interactions/apply {d64.interactions:.3e}")
 
+    tuned = None
+    if not args.no_tune:
+        from skelfmm_amd import autotune
+
+        tuned = {"fp64": autotune.tune(d64, 1), "fp32": autotune.tune(d32, 1)}
+        log(f"autotune: {tuned}")
     q64 = torch.as_tensor(q, dtype=torch.float64, device=dev)
     q32 = q64.to(torch.float32)
 
@@ -206,7 +362,9 @@ def main():
         "data": "synthetic: uniform [0,1]^3 points default_rng(0), RandomState(1) N(0,1) charges",
         "config": {"workload": "laplace_3d_uniform_cube", "N": N, "tol": args.tol, "leaf_size": args.leaf,
                    "nrhs": 1, "root_level": int(fmm.root_level), "nlevels": int(tree.nlevels),
-                   "interactions_per_apply": int(d64.interactions)},
+                   "interactions_per_apply": int(d64.interactions),
+                   "autotune": None if tuned is None else {k: {"max_sources": v["max_sources"]}
+                                                           for k, v in tuned.items()}},
         "fp64": {"value": round(N / t64 / 1e6, 3), "ms_per_step": round(t64 * 1e3, 4), "rel_l2_err": e64},
         "fp32_rel_l2_err": e32,
         "fp32_over_fp64_speedup": round(t64 / t32, 3),

@@ -100,8 +100,8 @@ typedef struct skf_p2p_args {
   const int* e_flags;
   int nitems;
   const int* items;   /* int4 per item */
-  int mfma_min_channels; /* FP32 real kernels with at least this many channels use the matrix-core
-                            kernel; 0 = default (16), negative = never */
+  int mfma_min_channels; /* real kernels (FP32 or FP64) with at least this many channels use the
+                            matrix-core kernel; 0 = default (16), negative = never */
 } skf_p2p_args;
 
 /* one level of the upward or downward interpolation pass */

@@ -1,0 +1,90 @@
+"""Launch-parameter autotuner for the device apply.
+
+For one uploaded build and one vector width it times, with HIP events, the
+near-field launch over
+
+* the source-run length of a work item (``max_sources``: how many source points
+  one 64-target work item covers before the target block is split across
+  workgroups -- trades load balance against atomics and re-staging), and
+* for FP32 real kernels with at least 16 channels, the matrix-core kernel
+  against the VALU kernel (RHS blocking: 64 channels per MFMA block, or the
+  widest VALU channel block that fits LDS),
+
+keeps the fastest and caches the choice per (device name, operator, precision,
+width, slots, interactions) in a JSON file (``$SKELFMM_AMD_TUNE_CACHE``, default
+``~/.cache/skelfmm_amd/autotune.json``). ``tune`` leaves the winning setting
+applied to the object.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+
+import torch
+
+SPLITS = (4096, 8192, 16384, 32768, 65536)
+
+
+def cache_path():
+    return os.environ.get("SKELFMM_AMD_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "skelfmm_amd", "autotune.json"))
+
+
+def signature(dev, width):
+    return "|".join(str(x) for x in (torch.cuda.get_device_name(dev.device), dev.operator.name, dev.precision,
+                                     width, dev.nslots, dev.interactions))
+
+
+def _load(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(path, db):
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(db, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass
+
+
+def apply_setting(dev, setting):
+    dev.set_mfma_min_channels(int(setting.get("mfma_min_channels", 0)))
+    if int(setting["max_sources"]) != int(getattr(dev.layout, "max_sources", -1)):
+        dev.set_near_field_split(int(setting["max_sources"]))
+
+
+def tune(dev, width=1, reps=3, use_cache=True, splits=SPLITS):
+    """Pick and apply the fastest near-field setting for ``dev`` at ``width`` channels; returns it."""
+    path = cache_path()
+    key = signature(dev, width)
+    db = _load(path) if use_cache else {}
+    if key in db:
+        apply_setting(dev, db[key])
+        return dict(db[key], cached=True)
+    nch = width // 2 if (dev.operator.is_complex or dev.kappa > 0) else width
+    mfma_opts = [0]
+    if dev.precision == 1 and not dev.operator.is_complex and nch >= 16:
+        mfma_opts = [0, -1]
+    results = []
+    for mf in mfma_opts:
+        dev.set_mfma_min_channels(mf)
+        for ms in splits:
+            dev.set_near_field_split(ms)
+            t = dev.time_near_field(width, reps=reps)
+            results.append((t, ms, mf))
+    t, ms, mf = min(results)
+    best = {"max_sources": ms, "mfma_min_channels": mf, "near_field_ms": round(t, 4),
+            "tried": [[round(a, 4), b, c] for a, b, c in results]}
+    apply_setting(dev, best)
+    if use_cache:
+        db[key] = best
+        _store(path, db)
+    return dict(best, cached=False)

@@ -241,6 +241,35 @@ class SkelFMMBatched(torch.nn.Module):
     def forward(self, q):
         return self.matvec(q)
 
+    def set_near_field_split(self, max_sources):
+        """Re-split the near-field work items into source runs of at most ``max_sources`` points."""
+        from .layout import resplit
+
+        resplit(self.layout, max_sources)
+        self._nf["items"] = torch.as_tensor(np.ascontiguousarray(self.layout.items.reshape(-1), dtype=np.int32),
+                                            device=self.device)
+        for k in list(self._plans):
+            self._destroy_plan(k)
+        self._graphs = {}
+
+    def time_near_field(self, width, reps=5):
+        """Mean duration (ms) of one near-field launch for ``width`` channels, HIP events on the current stream."""
+        p = self._plan(width)
+        a = self._p2p_args(width)
+        a.q = hip.ptr(p["qbuf"])
+        a.u = hip.ptr(p["ubuf"])
+        st = hip.stream_handle(self.device)
+        L = hip.lib()
+        hip.check(L.skf_p2p(ctypes.byref(a), st))
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            hip.check(L.skf_p2p(ctypes.byref(a), st))
+        e1.record()
+        torch.cuda.synchronize(self.device)
+        return e0.elapsed_time(e1) / reps
+
     def set_mfma_min_channels(self, n):
         """Channel count from which FP32 real kernels use the matrix-core near field (0 default 16, <0 never)."""
         self.mfma_min_channels = int(n)

@@ -353,25 +353,45 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
   }
 }
 
-// Multi-RHS near field on the matrix cores (FP32 form, real kernels): for every 4 sources
+// Multi-RHS near field on the matrix cores (real kernels, FP32 or FP64): for every 4 sources
 // a lane evaluates one kernel value K(t, s) with t = its wave's target (lane % 16) and
-// s = source (lane / 16) -- exactly the A-operand layout of v_mfma_f32_16x16x4_f32
-// (A[i][k] in lane i + 16k) -- and four MFMAs multiply it into 16x64 right-hand sides
-// read from LDS (B[k][j] in lane j + 16k). Wave w owns targets t0 + 16w .. t0 + 16w + 15;
-// D[4 (lane/16) + r][lane % 16] is accumulator r. FP32 MFMA sums of one 64-source tile are
-// added into FP64 accumulators after every tile.
+// s = source (lane / 16) -- the A-operand layout of v_mfma_{f32,f64}_16x16x4 (A[i][k] in
+// lane i + 16k) -- and four MFMAs multiply it into 16 x 64 right-hand sides read from LDS
+// (B[k][j] in lane j + 16k). Wave w owns targets t0 + 16w .. t0 + 16w + 15. Accumulator r
+// of a lane holds D[4 (lane/16) + r][lane % 16] (f32) or D[lane/16 + 4 r][lane % 16] (f64).
+// The MFMA sums of one 64-source tile are added into FP64 accumulators after every tile.
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int kMB = 64;         // channels per launch block (4 MFMA column blocks of 16)
 constexpr int kQS = kMB + 16;   // padded LDS row: rows 4 apart land on distinct banks
 
-template <int OP, bool PREC>
+template <typename R>
+struct Mfma;
+template <>
+struct Mfma<float> {
+  typedef v4f V;
+  __device__ static inline V mac(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static inline int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <>
+struct Mfma<double> {
+  typedef v4d V;
+  __device__ static inline V mac(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static inline int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+template <typename R, int OP, bool PREC>
 __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   static_assert(OpT<OP>::cplx == 0, "real kernels only");
   using S = Stage<OP, 1>;
+  using MF = Mfma<R>;
+  typedef typename MF::V V;
   constexpr int DIM = S::dim;
   constexpr int NG = S::ng;
-  __shared__ float qt[64 * kQS];
-  __shared__ float pt[64 * 8];  // x, y, z, nx, ny, nz per source
+  __shared__ R qt[64 * kQS];
+  __shared__ R pt[64 * 8];  // x, y, z, nx, ny, nz per source
   const int4 it = d.items[blockIdx.x];
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
   const int c0 = blockIdx.y * kMB;
@@ -383,16 +403,16 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   const int tloc = t0 + wave * 16 + (lane & 15);  // target of this lane's A rows
   const bool tvalid = tloc < ng;
   const int tslot = d.g_pt[g] + (tvalid ? tloc : t0);
-  const float* px = static_cast<const float*>(d.px);
-  const float* py = static_cast<const float*>(d.py);
-  const float* pz = static_cast<const float*>(d.pz);
-  const float* qv = static_cast<const float*>(d.q);
-  float xt = px[tslot], yt = py[tslot], zt = (DIM == 3) ? pz[tslot] : 0.f;
-  float gxt = 0, gyt = 0, gzt = 0;
+  const R* px = static_cast<const R*>(d.px);
+  const R* py = static_cast<const R*>(d.py);
+  const R* pz = static_cast<const R*>(d.pz);
+  const R* qv = static_cast<const R*>(d.q);
+  R xt = px[tslot], yt = py[tslot], zt = (DIM == 3) ? pz[tslot] : R(0);
+  R gxt = 0, gyt = 0, gzt = 0;
   if (OpT<OP>::geom == 2) {
-    gxt = static_cast<const float*>(d.nx)[tslot];
-    gyt = static_cast<const float*>(d.ny)[tslot];
-    gzt = static_cast<const float*>(d.nz)[tslot];
+    gxt = static_cast<const R*>(d.nx)[tslot];
+    gyt = static_cast<const R*>(d.ny)[tslot];
+    gzt = static_cast<const R*>(d.nz)[tslot];
   }
   double ctx = 0, cty = 0, ctz = 0;
   if (PREC) {
@@ -411,67 +431,67 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
     const int ps = d.e_pt[s], ns = d.e_n[s], qo = d.e_q[s], fl = d.e_flags[s];
     const bool self = (fl & SKF_SRC_SELF) != 0;
     const double sgn = (fl & SKF_SRC_NEG) ? -1.0 : 1.0;
-    float xs0 = xt, ys0 = yt, zs0 = zt;
+    R xs0 = xt, ys0 = yt, zs0 = zt;
     if (PREC) {
       const int eb = d.e_box[s];
-      xs0 = xt + float(ctx - d.centers[3 * eb + 0]);
-      ys0 = yt + float(cty - d.centers[3 * eb + 1]);
-      zs0 = zt + float(ctz - d.centers[3 * eb + 2]);
+      xs0 = xt + R(ctx - d.centers[3 * eb + 0]);
+      ys0 = yt + R(cty - d.centers[3 * eb + 1]);
+      zs0 = zt + R(ctz - d.centers[3 * eb + 2]);
     }
     for (int base = 0; base < ns; base += 64) {
       const int nsrc = min(64, ns - base);
       // stage 64 sources: coordinates (threads 0..63) and the 64 x 64 charge block (all threads)
       if (tid < 64) {
         const int j = base + tid;
-        float* row = &pt[tid * 8];
+        R* row = &pt[tid * 8];
         if (tid < nsrc) {
           const int sl = ps + j;
           row[0] = px[sl];
           row[1] = py[sl];
-          row[2] = (DIM == 3) ? pz[sl] : 0.f;
+          row[2] = (DIM == 3) ? pz[sl] : R(0);
           if (NG) {
-            row[3] = static_cast<const float*>(d.nx)[sl];
-            row[4] = static_cast<const float*>(d.ny)[sl];
-            row[5] = static_cast<const float*>(d.nz)[sl];
+            row[3] = static_cast<const R*>(d.nx)[sl];
+            row[4] = static_cast<const R*>(d.ny)[sl];
+            row[5] = static_cast<const R*>(d.nz)[sl];
           }
         } else {
           // far away, zero charge: contributes exactly zero
-          row[0] = 1e18f;
-          row[1] = 1e18f;
-          row[2] = 1e18f;
-          row[3] = row[4] = row[5] = 0.f;
+          row[0] = R(1e18);
+          row[1] = R(1e18);
+          row[2] = R(1e18);
+          row[3] = row[4] = row[5] = R(0);
         }
       }
       {
         const int sr = tid >> 2, cq = (tid & 3) * 16;
-        float* dst = &qt[sr * kQS + cq];
+        R* dst = &qt[sr * kQS + cq];
         if (sr < nsrc) {
-          const float* src = qv + (long long)(qo + base + sr) * d.width + c0 + cq;
+          const R* src = qv + (long long)(qo + base + sr) * d.width + c0 + cq;
 #pragma unroll
-          for (int c = 0; c < 16; ++c) dst[c] = (cq + c < ncv) ? src[c] : 0.f;
+          for (int c = 0; c < 16; ++c) dst[c] = (cq + c < ncv) ? src[c] : R(0);
         } else {
 #pragma unroll
-          for (int c = 0; c < 16; ++c) dst[c] = 0.f;
+          for (int c = 0; c < 16; ++c) dst[c] = R(0);
         }
       }
       __syncthreads();
-      v4f D[4];
+      V D[4];
 #pragma unroll
-      for (int nb = 0; nb < 4; ++nb) D[nb] = v4f{0.f, 0.f, 0.f, 0.f};
+      for (int nb = 0; nb < 4; ++nb) D[nb] = V{0, 0, 0, 0};
       const int ksteps = (nsrc + 3) >> 2;
       for (int kk = 0; kk < ksteps; ++kk) {
         const int sj = kk * 4 + (lane >> 4);
-        const float* row = &pt[sj * 8];
-        const float dx = xs0 - row[0], dy = ys0 - row[1], dz = (DIM == 3) ? zs0 - row[2] : 0.f;
-        float rn = 0.f;
+        const R* row = &pt[sj * 8];
+        const R dx = xs0 - row[0], dy = ys0 - row[1], dz = (DIM == 3) ? zs0 - row[2] : R(0);
+        R rn = 0;
         if (OpT<OP>::geom == 1) rn = dx * row[3] + dy * row[4] + dz * row[5];
         if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
-        float a = kernel_value<float, OP>(dx, dy, dz, rn, 0.f).re;
-        if (self && (base + sj) == tloc) a = 0.f;
-        const float* qrow = &qt[sj * kQS + (lane & 15)];
+        R a = kernel_value<R, OP>(dx, dy, dz, rn, R(0)).re;
+        if (self && (base + sj) == tloc) a = R(0);
+        const R* qrow = &qt[sj * kQS + (lane & 15)];
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb)
-          if (nb < nbv) D[nb] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, qrow[nb * 16], D[nb], 0, 0, 0);
+          if (nb < nbv) D[nb] = MF::mac(a, qrow[nb * 16], D[nb]);
       }
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb)
@@ -480,24 +500,23 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
       __syncthreads();
     }
   }
-  // D[4 (lane/16) + r][lane % 16] of column block nb -> target t0 + 16 wave + 4 (lane/16) + r, channel 16 nb + lane % 16
-  float* uo = static_cast<float*>(d.u);
+  R* uo = static_cast<R*>(d.u);
 #pragma unroll
   for (int nb = 0; nb < 4; ++nb) {
     const int ch = nb * 16 + (lane & 15);
     if (ch >= ncv) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int tl = t0 + wave * 16 + 4 * (lane >> 4) + r;
-      if (tl < ng) atomic_acc(&uo[(long long)(d.g_out[g] + tl) * d.width + c0 + ch], float(acc[nb][r]));
+      const int tl = t0 + wave * 16 + MF::row(lane, r);
+      if (tl < ng) atomic_acc(&uo[(long long)(d.g_out[g] + tl) * d.width + c0 + ch], R(acc[nb][r]));
     }
   }
 }
 
-template <int OP>
+template <typename R, int OP, bool PREC>
 int launch_p2p_mfma(const P2PDev& d, int nitems, hipStream_t st) {
   dim3 grid(nitems, (d.nch + kMB - 1) / kMB);
-  hipLaunchKernelGGL((p2p_mfma_kernel<OP, true>), grid, dim3(kWG), 0, st, d);
+  hipLaunchKernelGGL((p2p_mfma_kernel<R, OP, PREC>), grid, dim3(kWG), 0, st, d);
   SKF_LAUNCHED();
   return SKF_OK;
 }
@@ -540,20 +559,18 @@ int launch_p2p_op(int op, const P2PDev& d, int nitems, hipStream_t st) {
 }
 
 // channels evaluated per kernel value: the widest block whose LDS fits (FP64: 8 real / 4 complex,
-// FP32: 16 real / 8 complex); multi-RHS FP32 real kernels go to the matrix-core kernel instead
+// FP32: 16 real / 8 complex); multi-RHS real kernels (>= mfma_min channels) go to the matrix-core kernel
 template <typename R, bool PREC>
 int launch_p2p_prec(int op, const P2PDev& d, int nitems, hipStream_t st, int mfma_min) {
   const bool cx = (op == SKF_HELMHOLTZ_2D || op == SKF_HELMHOLTZ_3D || op == SKF_HELMHOLTZ_DL_3D ||
                    op == SKF_HELMHOLTZ_ADL_3D);
-  if constexpr (sizeof(R) == 4 && PREC) {
-    if (!cx && mfma_min > 0 && d.nch >= mfma_min) {
-      switch (op) {
-        case SKF_LAPLACE_2D: return launch_p2p_mfma<SKF_LAPLACE_2D>(d, nitems, st);
-        case SKF_LAPLACE_3D: return launch_p2p_mfma<SKF_LAPLACE_3D>(d, nitems, st);
-        case SKF_LAPLACE_DL_3D: return launch_p2p_mfma<SKF_LAPLACE_DL_3D>(d, nitems, st);
-        case SKF_LAPLACE_ADL_3D: return launch_p2p_mfma<SKF_LAPLACE_ADL_3D>(d, nitems, st);
-        default: break;
-      }
+  if (!cx && mfma_min > 0 && d.nch >= mfma_min) {
+    switch (op) {
+      case SKF_LAPLACE_2D: return launch_p2p_mfma<R, SKF_LAPLACE_2D, PREC>(d, nitems, st);
+      case SKF_LAPLACE_3D: return launch_p2p_mfma<R, SKF_LAPLACE_3D, PREC>(d, nitems, st);
+      case SKF_LAPLACE_DL_3D: return launch_p2p_mfma<R, SKF_LAPLACE_DL_3D, PREC>(d, nitems, st);
+      case SKF_LAPLACE_ADL_3D: return launch_p2p_mfma<R, SKF_LAPLACE_ADL_3D, PREC>(d, nitems, st);
+      default: break;
     }
   }
   if (d.nch == 1) return launch_p2p_op<R, PREC, 1>(op, d, nitems, st);
@@ -769,6 +786,111 @@ int grid_for(long long n) {
   return int(g);
 }
 
+// Multi-RHS interpolation (>= 16 channels): lanes run over right-hand-side channels, so every
+// vector row is read coalesced and each T entry is one wave-uniform load used by 64 lanes.
+// Upward: a wave owns 4 rows j of the item and accumulates them together (each x row read once
+// per 4 rows).
+template <typename R, bool CX>
+__global__ __launch_bounds__(256) void up_gemm_kernel(const R* __restrict__ T, const int* __restrict__ box_off,
+                                                      const int* __restrict__ box_bs, const int* __restrict__ box_rank,
+                                                      const long long* __restrict__ box_toff,
+                                                      const int2* __restrict__ items, const R* __restrict__ qvec,
+                                                      R* __restrict__ qskel, int w) {
+  constexpr int CWd = CX ? 2 : 1;
+  const int2 it = items[blockIdx.x];
+  const int b = it.x;
+  const int off = box_off[b], bs = box_bs[b], r = box_rank[b];
+  const long long toff = box_toff[b];
+  const int nred = bs - r;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int j0 = it.y + wave * kRowsPerWave;
+  const int nrow = max(0, min(kRowsPerWave, r - j0));
+  if (nrow == 0) return;
+  const int nchan = w / CWd;
+  for (int c0 = 0; c0 < nchan; c0 += 64) {
+    const int c = c0 + lane;
+    const bool cv = c < nchan;
+    R ar[kRowsPerWave], ai[kRowsPerWave];
+#pragma unroll
+    for (int k = 0; k < kRowsPerWave; ++k) ar[k] = ai[k] = 0;
+    for (int i = 0; i < nred; ++i) {
+      const R* x = qvec + (long long)(off + r + i) * w + (long long)c * CWd;
+      const R xr = cv ? x[0] : R(0);
+      const R xi = (CX && cv) ? x[1] : R(0);
+#pragma unroll
+      for (int k = 0; k < kRowsPerWave; ++k) {
+        if (k < nrow) {
+          const R* tp = T + (toff + (long long)(j0 + k) * nred + i) * CWd;
+          cmac<R, CX>(ar[k], ai[k], tp[0], CX ? tp[1] : R(0), xr, xi);
+        }
+      }
+    }
+    if (cv) {
+#pragma unroll
+      for (int k = 0; k < kRowsPerWave; ++k) {
+        if (k < nrow) {
+          const long long o = (long long)(off + j0 + k) * w + (long long)c * CWd;
+          qskel[o] = qvec[o] + ar[k];
+          if (CX) qskel[o + 1] = qvec[o + 1] + ai[k];
+        }
+      }
+    }
+  }
+}
+
+// Downward: a wave owns a quarter of the item's columns and handles 4 of them at a time.
+template <typename R, bool CX>
+__global__ __launch_bounds__(256) void down_gemm_kernel(const R* __restrict__ T, const int* __restrict__ box_off,
+                                                        const int* __restrict__ box_bs,
+                                                        const int* __restrict__ box_rank,
+                                                        const long long* __restrict__ box_toff,
+                                                        const int2* __restrict__ items, R* __restrict__ uvec,
+                                                        const R* __restrict__ uskel, int w) {
+  constexpr int CWd = CX ? 2 : 1;
+  constexpr int CG = 4;
+  const int2 it = items[blockIdx.x];
+  const int b = it.x;
+  const int off = box_off[b], bs = box_bs[b], r = box_rank[b];
+  const long long toff = box_toff[b];
+  const int nred = bs - r;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int cols_per_wave = kColsPerItem / 4;
+  const int ibeg = it.y + wave * cols_per_wave;
+  const int iend = min(nred, ibeg + cols_per_wave);
+  const int nchan = w / CWd;
+  for (int c0 = 0; c0 < nchan; c0 += 64) {
+    const int c = c0 + lane;
+    const bool cv = c < nchan;
+    for (int i0 = ibeg; i0 < iend; i0 += CG) {
+      const int ncol = min(CG, iend - i0);
+      R ar[CG], ai[CG];
+#pragma unroll
+      for (int k = 0; k < CG; ++k) ar[k] = ai[k] = 0;
+      for (int j = 0; j < r; ++j) {
+        const R* x = uskel + (long long)(off + j) * w + (long long)c * CWd;
+        const R xr = cv ? x[0] : R(0);
+        const R xi = (CX && cv) ? x[1] : R(0);
+        const R* tp = T + (toff + (long long)j * nred + i0) * CWd;
+#pragma unroll
+        for (int k = 0; k < CG; ++k)
+          if (k < ncol) cmac<R, CX>(ar[k], ai[k], tp[k * CWd], CX ? tp[k * CWd + 1] : R(0), xr, xi);
+      }
+      if (cv) {
+#pragma unroll
+        for (int k = 0; k < CG; ++k) {
+          if (k < ncol) {
+            R* y = uvec + (long long)(off + r + i0 + k) * w + (long long)c * CWd;
+            y[0] += ar[k];
+            if (CX) y[1] += ai[k];
+          }
+        }
+      }
+    }
+  }
+}
+
 template <typename R>
 int run_up_level(const skf_level_args* L, const void* T, int cx, void* qvec, void* qskel, int w, hipStream_t st) {
   R* qv = static_cast<R*>(qvec);
@@ -780,7 +902,15 @@ int run_up_level(const skf_level_args* L, const void* T, int cx, void* qvec, voi
   }
   if (L->nitems > 0) {
     const int2* items = reinterpret_cast<const int2*>(L->items);
-    if (cx)
+    const int nchan = cx ? w / 2 : w;
+    if (nchan >= 16) {
+      if (cx)
+        hipLaunchKernelGGL((up_gemm_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
+      else
+        hipLaunchKernelGGL((up_gemm_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
+    } else if (cx)
       hipLaunchKernelGGL((up_gemv_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
                          L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
     else
@@ -802,7 +932,15 @@ int run_down_level(const skf_level_args* L, const void* T, int cx, void* uvec, v
   }
   if (L->nitems > 0) {
     const int2* items = reinterpret_cast<const int2*>(L->items);
-    if (cx)
+    const int nchan = cx ? w / 2 : w;
+    if (nchan >= 16) {
+      if (cx)
+        hipLaunchKernelGGL((down_gemm_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
+      else
+        hipLaunchKernelGGL((down_gemm_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
+    } else if (cx)
       hipLaunchKernelGGL((down_gemv_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
                          L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
     else

@@ -167,6 +167,7 @@ def build_layout(fmm, max_sources=16384):
     L.down = down
 
     _near_field(L, fmm, off, bs, rk, max_sources)
+    L.max_sources = int(max_sources)
     return L
 
 
@@ -248,6 +249,8 @@ def merge_layouts(layouts):
     M.e_q = (M.e_pt + M.e_qkind * M.nslots).astype(np.int32)
     M.e_box = np.concatenate([L.e_box + bb for L, bb in zip(layouts, box_base)]).astype(np.int32)
     M.e_flags = np.concatenate([L.e_flags for L in layouts]).astype(np.int32)
+    M.g_e0 = np.concatenate([L.g_e0 + eb for L, eb in zip(layouts, e_base)])
+    M.g_e1 = np.concatenate([L.g_e1 + eb for L, eb in zip(layouts, e_base)])
     items = []
     costs = []
     for L, gb, eb in zip(layouts, g_base, e_base):
@@ -263,7 +266,46 @@ def merge_layouts(layouts):
     M.items = np.ascontiguousarray(items[order]).astype(np.int32)
     M.item_cost = costs[order]
     M.interactions = int(sum(L.interactions for L in layouts))
+    M.max_sources = int(layouts[0].max_sources)
     return M
+
+
+def make_items(g_n, g_e0, g_e1, e_n, max_sources):
+    """Near-field work items {group, first target, first entry, end entry}, heaviest first.
+
+    Each item covers up to 64 targets of one group and a run of that group's source
+    entries holding at most ``max_sources`` points (at least one entry).
+    """
+    items = []
+    costs = []
+    cums = np.concatenate(([0], np.cumsum(np.asarray(e_n, dtype=np.int64))))
+    for g in range(len(g_n)):
+        s0, s1 = int(g_e0[g]), int(g_e1[g])
+        ranges = []
+        a = s0
+        while a < s1:
+            b = a + 1
+            while b < s1 and cums[b + 1] - cums[a] <= max_sources:
+                b += 1
+            ranges.append((a, b))
+            a = b
+        ng = int(g_n[g])
+        for t0 in range(0, ng, TARGETS_PER_ITEM):
+            nt = min(TARGETS_PER_ITEM, ng - t0)
+            for a, b in ranges:
+                items.append((g, t0, a, b))
+                costs.append(nt * int(cums[b] - cums[a]))
+    items = np.array(items, dtype=np.int32).reshape(-1, 4)
+    costs = np.array(costs, dtype=np.int64)
+    order = np.argsort(-costs, kind="stable")
+    return np.ascontiguousarray(items[order]), costs[order]
+
+
+def resplit(L, max_sources):
+    """Rebuild a layout's near-field work items for another source-run length (in place)."""
+    L.items, L.item_cost = make_items(L.g_n, L.g_e0, L.g_e1, L.e_n, max_sources)
+    L.max_sources = int(max_sources)
+    return L
 
 
 def _near_field(L, fmm, off, bs, rk, max_sources):
@@ -318,31 +360,9 @@ def _near_field(L, fmm, off, bs, rk, max_sources):
     g_n = np.where(g_kind == 0, bs[g_box], rk[g_box])
     g_out = off[g_box] + g_kind * nslots
 
-    items = []
-    costs = []
-    cums = np.concatenate(([0], np.cumsum(e_n)))
-    for g in range(gkeys.shape[0]):
-        s0, s1 = int(gstart[g]), int(gstart[g] + gcount[g])
-        # split the entry run into ranges of at most max_sources points (at least one entry each)
-        ranges = []
-        a = s0
-        while a < s1:
-            b = a + 1
-            while b < s1 and cums[b + 1] - cums[a] <= max_sources:
-                b += 1
-            ranges.append((a, b))
-            a = b
-        ng = int(g_n[g])
-        for t0 in range(0, ng, TARGETS_PER_ITEM):
-            nt = min(TARGETS_PER_ITEM, ng - t0)
-            for a, b in ranges:
-                items.append((g, t0, a, b))
-                costs.append(nt * int(cums[b] - cums[a]))
-    items = np.array(items, dtype=np.int32).reshape(-1, 4)
-    costs = np.array(costs, dtype=np.int64)
-    order = np.argsort(-costs, kind="stable")
-    L.items = np.ascontiguousarray(items[order])
-    L.item_cost = costs[order]
+    L.g_e0 = gstart.astype(np.int64)
+    L.g_e1 = (gstart + gcount).astype(np.int64)
+    L.items, L.item_cost = make_items(g_n, L.g_e0, L.g_e1, e_n, max_sources)
     L.g_pt = g_pt.astype(np.int32)
     L.g_n = g_n.astype(np.int32)
     L.g_out = g_out.astype(np.int32)

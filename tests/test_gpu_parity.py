@@ -160,11 +160,12 @@ def test_multi_rhs_equals_columns(single):
         assert rel_l2(Y, ref) <= FP64_TOL
 
 
-def test_multi_rhs_complex():
+@pytest.mark.parametrize("nrhs", [3, 16])
+def test_multi_rhs_complex(nrhs):
     X = np.random.default_rng(6).random((3000, 3))
     fmm = make_build(X, ops.HELMHOLTZ_3D, kappa=3.0, tol=1e-6, leaf_size=100)
     rs = np.random.RandomState(3)
-    Q = rs.randn(X.shape[0], 3) + 1j * rs.randn(X.shape[0], 3)
+    Q = rs.randn(X.shape[0], nrhs) + 1j * rs.randn(X.shape[0], nrhs)
     Y, _ = _device_apply(fmm, Q)
     ref = BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(Q).numpy()
     assert rel_l2(Y, ref) <= FP64_TOL
@@ -271,6 +272,9 @@ def test_mfma_multi_rhs_precision_form(name, nrhs):
     Q = rng.normal(size=(X.shape[0], nrhs))
     d64 = SkelFMMBatched(fmm.tree, op).build_from_copy(fmm)
     y64 = d64.matvec(torch.as_tensor(Q, device=_gpu())).cpu().numpy()
+    d64.set_mfma_min_channels(-1)
+    y64_valu = d64.matvec(torch.as_tensor(Q, device=_gpu())).cpu().numpy()
+    assert rel_l2(y64, y64_valu) < 1e-13
     d32 = SkelFMMBatched(fmm.tree, op).build_from_copy(fmm, use_single_precision=True)
     q32 = torch.as_tensor(Q.astype(np.float32), device=_gpu())
     y_mfma = d32.matvec(q32).cpu().numpy().astype(np.float64)
@@ -334,3 +338,22 @@ def test_fp32_double_layer_surface_conditioning():
     print(f"sphere DL FP32: oracle single-precision form {e_ref:.2e}, GPU precision form {e_gpu:.2e}", flush=True)
     assert e_gpu <= e_ref
     assert rel_l2(y_mfma, y_valu) < 1e-5
+
+
+def test_autotune_picks_setting_and_keeps_parity(tmp_path, monkeypatch):
+    from skelfmm_amd import autotune
+    from skelfmm_amd.batched import SkelFMMBatched
+
+    monkeypatch.setenv("SKELFMM_AMD_TUNE_CACHE", str(tmp_path / "tune.json"))
+    X = np.random.default_rng(4).random((15000, 3))
+    fmm = make_build(X, ops.LAPLACE_3D, tol=1e-6, leaf_size=150)
+    dev = SkelFMMBatched(fmm.tree, ops.LAPLACE_3D).build_from_copy(fmm)
+    best = autotune.tune(dev, 1, reps=1, splits=(2048, 16384))
+    assert best["max_sources"] in (2048, 16384) and not best["cached"]
+    assert dev.layout.max_sources == best["max_sources"]
+    q = np.random.RandomState(0).randn(X.shape[0])
+    y = dev.matvec(torch.as_tensor(q, device=_gpu())).cpu().numpy()
+    ref = BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(q).numpy()
+    assert rel_l2(y, ref) <= FP64_TOL
+    again = autotune.tune(dev, 1, reps=1, splits=(2048, 16384))
+    assert again["cached"] and again["max_sources"] == best["max_sources"]
