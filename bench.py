@@ -37,7 +37,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 # flops per interaction (FMA = 2): dx,dy,dz (3) + r^2 (mul + 2 fma = 5) + rsqrt (1) + q*K accumulate (2)
-FLOPS_PER_INTERACTION = {"laplace_3d": 11, "laplace_2d": 11}
+FLOPS_PER_INTERACTION = {"laplace_3d": 11, "laplace_2d": 11,
+                         # + r = r^2 rsqrt (1), kappa r (1), sin and cos (2), e^{ikr}/r (2), complex q K (8)
+                         "helmholtz_3d": 23,
+                         # + (x - y).n (5) and the (ikr - 1) r^-2 factor (4)
+                         "helmholtz_double_layer_3d": 32}
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # HBM bytes per launch of the FP32 near-field kernel on the default workload, from the two
 # rocprofv3 --pmc passes summarised in profiles/apply_laplace3d_uniform1M_hbm_traffic.md
@@ -187,6 +191,180 @@ def run_annuli(args):
     print(json.dumps(line), flush=True)
 
 
+def mixture_points(n=2_000_000, seed=3, ncomp=16, sigma=0.02):
+    """Isotropic Gaussian mixture in [0,1]^3: default_rng(seed) draws the centres U[0,1]^3,
+    then Dirichlet(1) weights, then the component of every point, then the offsets; points
+    outside the cube are clipped to it (the count is returned)."""
+    rng = np.random.default_rng(seed)
+    c = rng.random((ncomp, 3))
+    w = rng.dirichlet(np.ones(ncomp))
+    comp = rng.choice(ncomp, size=n, p=w)
+    x = c[comp] + sigma * rng.standard_normal((n, 3))
+    clipped = int(np.sum(np.any((x < 0) | (x > 1), axis=1)))
+    return np.clip(x, 0.0, 1.0), clipped
+
+
+def run_helmholtz(args):
+    """3D Helmholtz workloads: ``sphere`` (1M points on the unit sphere, kappa 10 pi, tol 1e-5,
+    leaf 256, single or double layer) and ``mixture`` (2M-point Gaussian mixture, kappa 20,
+    tol 1e-6, leaf 256) whose build runs twice on the GPU: adaptive IDs, then learned-rank
+    sketched IDs with adaptive fallback. Value: complex64 apply throughput."""
+    import torch
+
+    from oracle.direct import direct_sum_c
+    from oracle.kernels import GEOM_SIDE
+    from skelfmm_amd import hip, learned_rank, operators
+    from skelfmm_amd.batched import SkelFMMBatched
+    from skelfmm_amd.build import RecursiveSkeletonization
+    from skelfmm_amd.tree import BalancedTree
+
+    hip.require_gpu()
+    dev = torch.device("cuda", 0)
+    extra = {}
+    if args.workload == "sphere":
+        N = args.n
+        v = np.random.default_rng(0).normal(size=(N, 3))
+        X = v / np.linalg.norm(v, axis=1, keepdims=True)
+        op = operators.HELMHOLTZ_DOUBLE_LAYER_3D if args.double_layer else operators.HELMHOLTZ_3D
+        geom = X.copy() if args.double_layer else None
+        kappa, tol, leaf = 10 * np.pi, 1e-5, 256
+        workload = "helmholtz_3d_sphere" + ("_double_layer" if args.double_layer else "")
+        data = "synthetic: unit-sphere points default_rng(0) normal/|.|, normals = points"
+    else:
+        N = 2_000_000 if args.n == 1_000_000 else args.n
+        X, clipped = mixture_points(N)
+        op, geom = operators.HELMHOLTZ_3D, None
+        kappa, tol, leaf = 20.0, 1e-6, 256
+        workload = "helmholtz_3d_gaussian_mixture"
+        data = f"synthetic: 16-component Gaussian mixture default_rng(3), sigma 0.02, {clipped} points clipped to [0,1]^3"
+        extra["clipped_points"] = clipped
+    rs = np.random.RandomState(1)
+    q = rs.randn(N) + 1j * rs.randn(N)
+
+    t0 = time.perf_counter()
+    tree = BalancedTree(X, leaf_size=leaf)
+    t_tree = time.perf_counter() - t0
+    log(f"{workload}: N={N}, tree {t_tree:.1f}s, levels {tree.nlevels}, boxes {tree.nboxes}")
+
+    def build(rank_model):
+        f = RecursiveSkeletonization(tree, op, kappa=kappa, tol=tol, geometry=geom)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        f.skel_tree(npoints_max=500, device="gpu", rank_model=rank_model)
+        torch.cuda.synchronize()
+        t_id = time.perf_counter() - ts
+        ts = time.perf_counter()
+        f.setup_lists()
+        d = SkelFMMBatched(tree, op, kappa=kappa).build_from_copy(f, use_single_precision=False)
+        torch.cuda.synchronize()
+        return f, d, t_id, time.perf_counter() - ts
+
+    builds = {}
+    f_a, d_a, tid_a, tset_a = build(None)
+    builds["adaptive"] = {"id_s": round(tid_a, 3), "lists_upload_s": round(tset_a, 2),
+                          "build_s": round(t_tree + tid_a + tset_a, 2), "rank_sum": int(f_a.rank_list.sum()),
+                          "level_s": {int(k): round(v, 3) for k, v in f_a.level_times.items()}}
+    log(f"adaptive GPU build: IDs {tid_a:.2f}s, lists+upload {tset_a:.1f}s, root level {f_a.root_level}")
+    f_use, d64 = f_a, d_a
+    model = learned_rank.load_default() if args.workload == "mixture" else None
+    if model is not None:
+        # warm the sketch path's library calls once outside the timing (rocBLAS handle set-up)
+        f_l, d_l, tid_l, tset_l = build(model)
+        st = f_l.gpu_stats
+        builds["learned"] = {"id_s": round(tid_l, 3), "lists_upload_s": round(tset_l, 2),
+                             "build_s": round(t_tree + tid_l + tset_l, 2), "rank_sum": int(f_l.rank_list.sum()),
+                             "stats": st, "fallback_rate": round(st["fallback"] / max(1, st["sketched"] + st["fallback"]), 4),
+                             "level_s": {int(k): round(v, 3) for k, v in f_l.level_times.items()}}
+        builds["id_speedup_learned_over_adaptive"] = round(tid_a / tid_l, 3)
+        log(f"learned GPU build: IDs {tid_l:.2f}s ({tid_a / tid_l:.2f}x), stats {st}")
+        f_use, d64 = f_l, d_l
+    d32 = SkelFMMBatched(tree, op, kappa=kappa).build_from_copy(f_use, use_single_precision=True)
+
+    q128 = torch.as_tensor(q, dtype=torch.complex128, device=dev)
+    q64 = q128.to(torch.complex64)
+
+    def timed(d, x):
+        y = torch.empty_like(x)
+        for _ in range(args.warmup):
+            d.apply_device(x, out=y)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(args.steps):
+            d.apply_device(x, out=y)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - ts) / args.steps, y
+
+    t64, y64 = timed(d32, q64)
+    t128, y128 = timed(d64, q128)
+    p64 = d32.time_near_field(2, reps=5) * 1e-3
+    p128 = d64.time_near_field(2, reps=5) * 1e-3
+    log(f"apply complex64 {t64 * 1e3:.2f} ms, complex128 {t128 * 1e3:.2f} ms; near field {p64 * 1e3:.2f} / "
+        f"{p128 * 1e3:.2f} ms")
+
+    idx = np.random.default_rng(0).choice(N, min(args.ncheck, N), replace=False)
+    side = GEOM_SIDE.get(op.name)
+    ref = direct_sum_c(op.name, X[idx], X, q, kappa=kappa, geom_trg=geom[idx] if side == "target" else None,
+                       geom_src=geom if side == "source" else None)
+    err = lambda y: float(np.linalg.norm(y.cpu().numpy()[idx] - ref) / np.linalg.norm(ref))
+    e64, e128 = err(y64), err(y128)
+    errs = {"complex64": e64, "complex128": e128}
+    if "learned" in builds:
+        _, ya = timed(d_a, q128)
+        errs["complex128_adaptive_build"] = err(ya)
+    log(f"rel-l2 vs direct sum at {idx.size} targets: {errs}")
+
+    fpi = FLOPS_PER_INTERACTION[op.name]
+    tf = d32.interactions * fpi / p64 / 1e12
+    line = {
+        "metric": "Mpoints*RHS/s per apply, complex64 precision form (complex128 form and build s beside it)",
+        "value": round(N / t64 / 1e6, 3),
+        "unit": "Mpoints*RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t64 * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64",
+        "data": data + "; charges RandomState(1) randn + 1j randn",
+        "config": {"workload": workload, "N": N, "kappa": kappa, "tol": tol, "leaf_size": leaf, "nrhs": 1,
+                   "nlevels": int(tree.nlevels), "root_level": int(f_use.root_level),
+                   "interactions_per_apply": int(d32.interactions), **extra},
+        "complex128": {"value": round(N / t128 / 1e6, 3), "ms_per_step": round(t128 * 1e3, 4)},
+        "rel_l2_err": errs,
+        "builds": builds,
+        "roofline": {"bound": "mfma", "achieved": round(tf, 3), "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                     "frac": round(tf / PEAK_TFLOPS["fp32"], 4), "traffic": None,
+                     "compute_unit": "VALU (per-pair kernel evaluation with sin/cos)",
+                     "kernel": f"near field complex64 ({op.name})", "kernel_ms": round(p64 * 1e3, 4),
+                     "flops_per_interaction": fpi},
+        "cpu_baseline": _cpu_baseline(f_use, q, d64.interactions, N, args) if not args.no_cpu else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def _cpu_baseline(fmm, q, interactions, N, args, fraction=0.01):
+    """Padded torch-CPU restatement of the reference apply on a sample of every pair list."""
+    import torch
+
+    from oracle.batched_apply import BatchedApplyCPU
+
+    ncores = min(16, len(os.sched_getaffinity(0)))
+    torch.set_num_threads(ncores)
+    ba = BatchedApplyCPU(fmm, bytes_budget=10e9, pair_fraction=fraction)
+    tc = time.perf_counter()
+    ba.matvec(q)
+    el = time.perf_counter() - tc
+    est = el * interactions / max(ba.interactions, 1)
+    cpu = {"value": round(N / est / 1e6, 6), "unit": "Mpoints*RHS/s", "cores": ncores, "kind": "port",
+           "sample": (f"padded torch-CPU complex128 restatement of the reference apply on {fraction:.0%} of the pairs "
+                      f"of every interaction list ({ba.interactions:.3e} interactions) plus full interpolation "
+                      f"passes, {el:.1f}s, extrapolated to {interactions:.3e} interactions")}
+    log(f"cpu baseline: {cpu}")
+    return cpu
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,8 +378,11 @@ def main():
     ap.add_argument("--ncheck", type=int, default=1000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 512 MiB device-to-device copy first (calibrates rocprofv3 byte counters)")
-    ap.add_argument("--workload", choices=("laplace3d_cube", "annuli"), default="laplace3d_cube",
-                    help="laplace3d_cube: 1M-point 3D Laplace (default); annuli: 32 batched 2D annuli x 64 RHS")
+    ap.add_argument("--workload", choices=("laplace3d_cube", "annuli", "sphere", "mixture"), default="laplace3d_cube",
+                    help="laplace3d_cube: 1M-point 3D Laplace (default); annuli: 32 batched 2D annuli x 64 RHS; "
+                         "sphere: 1M-point 3D Helmholtz on the sphere; mixture: 2M-point Gaussian mixture, "
+                         "learned-rank vs adaptive GPU build")
+    ap.add_argument("--double-layer", action="store_true", help="sphere workload: Helmholtz double layer")
     ap.add_argument("--instances", type=int, default=32)
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="skip the near-field autotuner (on by default)")
@@ -212,6 +393,8 @@ def main():
         sys.exit(2)
     if args.workload == "annuli":
         return run_annuli(args)
+    if args.workload in ("sphere", "mixture"):
+        return run_helmholtz(args)
 
     import torch
 

@@ -174,6 +174,50 @@ int skf_p2p_blocks(int kernel, int precision, int width, double kappa, int nbloc
                    const void* xt, const void* xs, const void* nt, const void* ns, const void* q, void* u,
                    int self_interaction, void* stream);
 
+/* ---------------------------------------------------------------- GPU build
+ * Reference: the per-box interpolative decomposition of the build,
+ * skelfmm/util.py:87-124 (build_stacked_proxy_matrix) and util.py:13-51
+ * (_skeleton_id_pivoted_qr), batched over the boxes of one tree level
+ * (the per-box loop of skelfmm/recursive_skel.py:412-491).
+ * Complex (Helmholtz) matrices are interleaved (re, im) doubles.
+ */
+const char* skf_build_last_error(void);
+
+/* A_b (row-major m_b x n_b with row stride lda, or n_b when lda = 0, at a_off[b] elements):
+   rows = proxies, columns = the box's candidate
+   points idx[idx_off[b] .. + ncols[b]] of pts (N x 3, z = 0 in 2D); proxies = centers[b] +
+   side[b] * unit_proxy (nproxy x 3); stacked = 1 for [K_PB; K_BP^T] (m_b = 2 nproxy) with outward
+   proxy normals, else K_BP^T (m_b = nproxy). */
+int skf_proxy_matrices(int kernel, int ndim, double kappa, const double* pts, const double* normals, const int* idx,
+                       const long long* idx_off, const int* ncols, const double* centers, const double* side,
+                       const double* unit_proxy, int nproxy, int stacked, int complex_out, void* A,
+                       const long long* a_off, int lda, int nbox, int max_cols, void* stream);
+
+/* In-place column-pivoted Householder QR of every A_b for at most kmax[b] steps, stopping at the
+   first |R_ii| <= tol |R_00|: rank[b] = that i (converged[b] = 1), or kmax[b] with converged[b] = 0
+   when the cutoff was not reached within kmax[b] < min(m, n) steps. perm (at p_off[b]) holds the
+   column pivots, rdiag the |R_ii|. */
+int skf_pqr_batched(void* A, const long long* a_off, int lda, const int* m, const int* n, const int* kmax,
+                    double tol, int* rank, int* converged, int* perm, const long long* p_off, double* rdiag, int nbox,
+                    int max_m, int max_n, int complex_a, void* stream);
+
+/* T_b = R11^{-1} R12 (rank x (n - rank), row-major at t_off[b]) from the factored A_b. */
+int skf_trsm_batched(const void* A, const long long* a_off, int lda, const int* n, const int* rank, void* T,
+                     const long long* t_off, int nbox, int max_cols, int complex_a, void* stream);
+
+/* ------------------------------------------------- learned rank predictor
+ * No counterpart in the reference (which always runs the full adaptive ID):
+ * an 8 -> 64 -> 64 -> 1 GELU MLP (FP32) mapping box features to skeleton rank.
+ * params: W1[64][8], b1[64], W2[64][64], b2[64], w3[64], b3 (skf_mlp_param_count() floats).
+ */
+int skf_mlp_param_count(void);
+int skf_mlp_forward(const float* params, const float* X, int n, float* y, void* stream);
+/* adds d(mean squared error)/d(params) into grad and the mean squared error into *loss */
+int skf_mlp_backward(const float* params, const float* X, const float* target, int n, float* grad, float* loss,
+                     void* stream);
+int skf_adam_step(float* params, const float* grad, float* m1, float* m2, int np, float lr, float beta1, float beta2,
+                  float eps, int step, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,7 +8,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "skelfmm_hip.hip")
+SRC = [os.path.join(HERE, "csrc", "skelfmm_hip.hip"), os.path.join(HERE, "csrc", "skelfmm_build.hip")]
 OUT = os.path.join(HERE, "libskelfmm_hip.so")
 
 HIPCC_FLAGS = [
@@ -31,7 +31,7 @@ def hipcc():
 
 def build_library(verbose=False):
     tmp = OUT + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), SRC, "-o", tmp]
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include")] + SRC + ["-o", tmp]
     if verbose:
         print(" ".join(cmd))
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -211,6 +211,30 @@ class RecursiveSkeletonization:
                                stack_symmetric=self.stack_symmetric)
         return skeleton_id_pivoted_qr(K, self.tol)
 
+    def _skel_level_gpu(self, gpu, lev, boxes, centers, length, nproxy, rank_model):
+        cands = [self.cand[b] for b in boxes]
+        res = [None] * len(boxes)
+        if rank_model is not None:
+            from .learned_rank import box_features
+
+            F = box_features(self, lev, boxes, centers, length)
+            k_pred = rank_model.predict_rank(F)
+            res = gpu.sketched(cands, centers, length, nproxy, k_pred)
+            self.gpu_stats["sketched"] += sum(r is not None for r in res)
+        todo = [i for i, r in enumerate(res) if r is None]
+        if todo:
+            if rank_model is not None:
+                self.gpu_stats["fallback"] += len(todo)
+            sub = gpu.adaptive([cands[i] for i in todo], centers[todo], length, nproxy)
+            for i, r in zip(todo, sub):
+                res[i] = r
+            self.gpu_stats["adaptive"] += sum(r is not None for r in sub)
+        for i, r in enumerate(res):
+            if r is None:  # workspace beyond one workgroup's LDS: host ID
+                res[i] = self._skel_one(boxes[i], centers[i], length, nproxy)
+                self.gpu_stats["cpu"] += 1
+        return res
+
     def set_skel_box_info(self, box, rank, idx, proj):
         """Permute the box skeleton-first and store ``T`` (``recursive_skel.py:264-324``)."""
         self.rank_list[box] = rank
@@ -219,11 +243,25 @@ class RecursiveSkeletonization:
             self.child_src[box] = self.child_src[box][idx]
         self.proj[box] = proj
 
-    def skel_tree(self, npoints_max=500, p="auto", verbose=False, min_parallel_items=64, nworkers=None):
-        """Bottom-up skeletonization until level 2 or ``sum(bs) <= npoints_max`` (``recursive_skel.py:326-502``)."""
+    def skel_tree(self, npoints_max=500, p="auto", verbose=False, min_parallel_items=64, nworkers=None,
+                  device="cpu", rank_model=None):
+        """Bottom-up skeletonization until level 2 or ``sum(bs) <= npoints_max`` (``recursive_skel.py:326-502``).
+
+        ``device="gpu"`` runs the per-box IDs of a level as batched GPU kernels
+        (``skelfmm_amd.gpu_build``): the adaptive pivoted QR, or -- with ``rank_model``
+        (a trained ``learned_rank.RankMLP``) -- the sketched fixed-rank QR with the
+        proxy-residual check, falling back to the adaptive QR box by box.
+        """
         if nworkers is None:
             nworkers = _default_workers()
         self.root_level = max(0, self.tree.nlevels - 1)
+        self.build_device = device
+        self.gpu_stats = {"sketched": 0, "fallback": 0, "adaptive": 0, "cpu": 0}
+        gpu = None
+        if device == "gpu":
+            from .gpu_build import GPUSkeletonizer
+
+            gpu = GPUSkeletonizer(self.operator, self.kappa, self.tol, self.tree.XX, self.geometry)
         pool = None
         try:
             for lev in range(self.tree.nlevels - 1, 0, -1):
@@ -241,7 +279,9 @@ class RecursiveSkeletonization:
                 centers = self.tree.get_boxes_centers(boxes_lev)
                 nproxy = int(self.nproxy_level[lev])
                 parallel = (p is not None) and nworkers > 1 and boxes_lev.shape[0] >= min_parallel_items
-                if parallel:
+                if gpu is not None and self.tol < 1:
+                    res = self._skel_level_gpu(gpu, lev, boxes_lev, centers, length, nproxy, rank_model)
+                elif parallel:
                     # one BLAS thread per worker: the QRs run concurrently (LAPACK releases the GIL)
                     if pool is None:
                         pool = ThreadPoolExecutor(max_workers=nworkers)

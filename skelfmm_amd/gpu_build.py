@@ -1,0 +1,262 @@
+"""GPU interpolative decompositions for the skeletonization build.
+
+Two paths for the boxes of one tree level, both in FP64 (complex128 for Helmholtz):
+
+* ``adaptive``: proxy matrices ``A_b`` (``skf_proxy_matrices``), column-pivoted
+  Householder QR of each ``A_b`` stopped at the first ``|R_ii| <= tol |R_00|``
+  (``skf_pqr_batched``) and ``T_b = R11^{-1} R12`` (``skf_trsm_batched``): the
+  reference's adaptive ID (``skelfmm/util.py:13-51``) with early stopping instead
+  of a full factorization.
+* ``sketched`` (the learned-rank build): given a predicted rank ``k_b`` per box, a
+  Gaussian sketch ``Y_b = Omega A_b`` with ``l_b = ceil(1.25 k_b) + 8`` rows (one
+  batched GEMM) replaces ``A_b`` in the pivoted QR, which then runs on ``l_b`` rows
+  instead of ``m``. A box is accepted when the QR of its sketch reaches the rank
+  cutoff within ``l_b`` steps and an independent 8-row sketch of the residual
+  ``A[:, red] - A[:, skel] T`` stays within ``10 tol`` of the sketched block norm
+  (the proxy-residual check); every other box falls back to the adaptive path.
+
+Results are returned in the CPU build's convention: ``(rank, idx, proj)`` with
+``idx`` the column pivots and ``proj = T``.
+"""
+
+from __future__ import annotations
+
+import math
+from time import perf_counter
+
+import numpy as np
+import torch
+
+from . import hip
+from .proxy import get_cube_surface, get_sphere_surface
+
+__all__ = ["GPUSkeletonizer", "accept_sketch"]
+
+LDS_BYTES = 64 * 1024
+
+
+def _unit_proxy(operator, ndim, nproxy):
+    """Proxy points of a box of side 1 / diameter 1 centred at the origin, as (P, 3)."""
+    if operator.proxy_surface == "cube":
+        u = get_cube_surface(np.zeros(ndim), 1.0, nproxy)
+    else:
+        u = get_sphere_surface(np.zeros(3), 1.0, nproxy)
+    out = np.zeros((u.shape[0], 3))
+    out[:, :u.shape[1]] = u
+    return out
+
+
+def accept_sketch(Z, result, lrows, tol, cap=None, slack=10.0):
+    """Acceptance test of a sketched ID ``result = (rank, idx, T)`` from ``lrows`` sketch rows.
+
+    Rejected when the sketch's QR ran out of rows before the rank cutoff
+    (``rank >= lrows``: the predicted rank was too low -- unless ``lrows``
+    already reaches ``cap = min(m, n)``, the largest possible rank), or when the residual
+    ``Z[:, red] - Z[:, skel] T`` of an independent sketch ``Z`` of the proxy
+    matrix has a column norm above ``slack * tol`` times the largest column norm
+    of ``Z`` (the proxy-residual check at ``tol``).
+    """
+    rk, idx, T = result
+    if rk >= lrows and (cap is None or lrows < cap):
+        return False
+    n = Z.shape[1]
+    if n == 0 or rk == n:
+        return True
+    skel, red = Z[:, idx[:rk]], Z[:, idx[rk:]]
+    E = red - skel @ T if rk > 0 else red
+    den = float(np.max(np.linalg.norm(Z, axis=0)))
+    return float(np.max(np.linalg.norm(E, axis=0))) <= slack * tol * max(den, 1e-300)
+
+
+class GPUSkeletonizer:
+    """Batched GPU IDs for one build (operator, kappa, tolerance, points, normals)."""
+
+    def __init__(self, operator, kappa, tol, points, geometry=None, device=None, budget_bytes=24e9, seed=0):
+        hip.require_gpu()
+        self.L = hip.lib()
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+        self.op = operator
+        self.kappa = float(kappa)
+        self.tol = float(tol)
+        self.ndim = points.shape[1]
+        P = np.zeros((points.shape[0], 3))
+        P[:, :self.ndim] = points
+        self.pts = torch.as_tensor(P, dtype=torch.float64, device=self.device).contiguous()
+        self.nrm = None
+        if geometry is not None:
+            self.nrm = torch.as_tensor(np.asarray(geometry, dtype=np.float64), device=self.device).contiguous()
+        self.stacked = not (operator.is_symmetric and geometry is None)
+        self.cplx = bool(operator.is_complex)
+        self.budget = float(budget_bytes)
+        self.gen = torch.Generator(device=self.device)
+        self.gen.manual_seed(seed)
+        self.timings = {}
+
+    # ----------------------------------------------------------------- common
+    def _stream(self):
+        return hip.stream_handle(self.device)
+
+    def _chunks(self, ncols, rows):
+        """Box index chunks whose matrices fit the byte budget."""
+        es = 16 if self.cplx else 8
+        out, cur, acc = [], [], 0
+        for i, n in enumerate(ncols):
+            b = rows * int(n) * es
+            if cur and (acc + b > self.budget or len(cur) >= 32768):  # grid.y stays below 65536
+                out.append(cur)
+                cur, acc = [], 0
+            cur.append(i)
+            acc += b
+        if cur:
+            out.append(cur)
+        return out
+
+    def _proxy(self, cands, centers, length, nproxy, lda=0):
+        """Proxy matrices of the given boxes; returns (A, a_off, m, unit count)."""
+        unit = _unit_proxy(self.op, self.ndim, nproxy)
+        P = unit.shape[0]
+        m = 2 * P if self.stacked else P
+        ncols = np.array([len(c) for c in cands], dtype=np.int32)
+        idx = np.concatenate(cands).astype(np.int32)
+        idx_off = np.concatenate(([0], np.cumsum(ncols)[:-1])).astype(np.int64)
+        if lda > 0:
+            a_off = (np.arange(len(cands), dtype=np.int64) * m * lda)
+            total = len(cands) * m * lda
+        else:
+            a_off = np.concatenate(([0], np.cumsum(m * ncols.astype(np.int64))[:-1])).astype(np.int64)
+            total = int(m * ncols.astype(np.int64).sum())
+        dev = self.device
+        A = torch.zeros(total * (2 if self.cplx else 1), dtype=torch.float64, device=dev)
+        c3 = np.zeros((len(cands), 3))
+        c3[:, :self.ndim] = centers
+        t_idx = torch.as_tensor(idx, device=dev)
+        t_ioff = torch.as_tensor(idx_off, device=dev)
+        t_n = torch.as_tensor(ncols, device=dev)
+        t_c = torch.as_tensor(c3, device=dev)
+        t_side = torch.full((len(cands),), 2.95 * length, dtype=torch.float64, device=dev)
+        t_unit = torch.as_tensor(unit, device=dev)
+        t_aoff = torch.as_tensor(a_off, device=dev)
+        hip.check_build(self.L.skf_proxy_matrices(
+            self.op.kernel_id, self.ndim, self.kappa, hip.ptr(self.pts), hip.ptr(self.nrm), hip.ptr(t_idx),
+            hip.ptr(t_ioff), hip.ptr(t_n), hip.ptr(t_c), hip.ptr(t_side), hip.ptr(t_unit), P,
+            1 if self.stacked else 0, 1 if self.cplx else 0, hip.ptr(A), hip.ptr(t_aoff), int(lda), len(cands),
+            int(ncols.max()), self._stream()))
+        keep = (t_idx, t_ioff, t_n, t_c, t_side, t_unit)
+        return A, t_aoff, a_off, m, ncols, keep
+
+    def _qr_trsm(self, A, t_aoff, lda, rows, ncols, kmax):
+        """Batched pivoted QR (early stop at the cutoff) + triangular solves; host results."""
+        dev = self.device
+        nb = len(ncols)
+        m_arr = np.asarray(rows, dtype=np.int32)
+        t_m = torch.as_tensor(m_arr, device=dev)
+        t_n = torch.as_tensor(np.asarray(ncols, dtype=np.int32), device=dev)
+        t_k = torch.as_tensor(np.asarray(kmax, dtype=np.int32), device=dev)
+        p_off = np.concatenate(([0], np.cumsum(ncols)[:-1])).astype(np.int64)
+        t_poff = torch.as_tensor(p_off, device=dev)
+        perm = torch.empty(int(np.sum(ncols)), dtype=torch.int32, device=dev)
+        rdiag = torch.zeros(int(np.sum(ncols)), dtype=torch.float64, device=dev)
+        rank = torch.empty(nb, dtype=torch.int32, device=dev)
+        conv = torch.empty(nb, dtype=torch.int32, device=dev)
+        max_m, max_n = int(m_arr.max()), int(np.max(ncols))
+        hip.check_build(self.L.skf_pqr_batched(
+            hip.ptr(A), hip.ptr(t_aoff), int(lda), hip.ptr(t_m), hip.ptr(t_n), hip.ptr(t_k), self.tol, hip.ptr(rank),
+            hip.ptr(conv), hip.ptr(perm), hip.ptr(t_poff), hip.ptr(rdiag), nb, max_m, max_n, 1 if self.cplx else 0,
+            self._stream()))
+        r_h = rank.cpu().numpy().astype(np.int64)
+        c_h = conv.cpu().numpy().astype(bool)
+        nc = np.maximum(np.asarray(ncols, dtype=np.int64) - r_h, 0)
+        sizes = np.where(c_h, r_h * nc, 0)
+        t_off = np.concatenate(([0], np.cumsum(sizes)[:-1])).astype(np.int64)
+        T = torch.zeros(max(1, int(sizes.sum())) * (2 if self.cplx else 1), dtype=torch.float64, device=dev)
+        rank_solve = torch.as_tensor(np.where(c_h, r_h, 0).astype(np.int32), device=dev)
+        t_toff = torch.as_tensor(t_off, device=dev)
+        hip.check_build(self.L.skf_trsm_batched(
+            hip.ptr(A), hip.ptr(t_aoff), int(lda), hip.ptr(t_n), hip.ptr(rank_solve), hip.ptr(T), hip.ptr(t_toff), nb,
+            max(1, int(nc.max()) if nc.size else 1), 1 if self.cplx else 0, self._stream()))
+        perm_h = perm.cpu().numpy().astype(np.int64)
+        T_h = T.cpu().numpy()
+        if self.cplx:
+            T_h = T_h.view(np.complex128)
+        out = []
+        for i in range(nb):
+            n = int(ncols[i])
+            r = int(r_h[i])
+            idx = perm_h[p_off[i]:p_off[i] + n]
+            if not c_h[i]:
+                out.append(None)
+                continue
+            proj = T_h[t_off[i]:t_off[i] + r * (n - r)].reshape(r, n - r) if r * (n - r) > 0 else \
+                np.zeros((r, n - r), dtype=np.complex128 if self.cplx else np.float64)
+            out.append((r, idx, np.ascontiguousarray(proj)))
+        return out
+
+    def _lds_ok(self, rows, n):
+        es = 16 if self.cplx else 8
+        return 16 * n + es * rows <= LDS_BYTES
+
+    # ---------------------------------------------------------------- adaptive
+    def adaptive(self, cands, centers, length, nproxy):
+        """Adaptive IDs of a level's boxes (None where the QR workspace does not fit LDS)."""
+        t0 = perf_counter()
+        unitP = _unit_proxy(self.op, self.ndim, nproxy).shape[0]
+        m = 2 * unitP if self.stacked else unitP
+        results = [None] * len(cands)
+        ok = [i for i, c in enumerate(cands) if self._lds_ok(m, len(c))]
+        for ch in self._chunks([len(cands[i]) for i in ok], m):
+            sel = [ok[i] for i in ch]
+            A, t_aoff, _, m_, ncols, keep = self._proxy([cands[i] for i in sel], centers[sel], length, nproxy)
+            kmax = np.minimum(m_, ncols)
+            res = self._qr_trsm(A, t_aoff, 0, np.full(len(sel), m_), ncols, kmax)
+            for i, r in zip(sel, res):
+                results[i] = r
+            del A, keep
+        self.timings["adaptive"] = self.timings.get("adaptive", 0.0) + perf_counter() - t0
+        return results
+
+    # ---------------------------------------------------------------- sketched
+    def sketched(self, cands, centers, length, nproxy, k_pred, oversample=1.25, extra=8, check_rows=8):
+        """Learned-rank IDs from Gaussian sketches; None where the box must fall back."""
+        t0 = perf_counter()
+        unitP = _unit_proxy(self.op, self.ndim, nproxy).shape[0]
+        m = 2 * unitP if self.stacked else unitP
+        nb = len(cands)
+        ncols_all = np.array([len(c) for c in cands])
+        lrows = np.minimum(np.minimum(ncols_all, m), np.ceil(oversample * np.asarray(k_pred)).astype(np.int64) + extra)
+        lrows = np.maximum(lrows, 1)
+        results = [None] * nb
+        ok = [i for i in range(nb) if self._lds_ok(m, int(ncols_all[i]))]
+        # group boxes of similar width so the padded batch wastes little
+        order = sorted(ok, key=lambda i: ncols_all[i])
+        groups, cur = [], []
+        es = 16 if self.cplx else 8
+        for i in order:
+            grow = (len(cur) + 1) * m * int(ncols_all[i]) * es
+            if cur and (ncols_all[i] > 1.25 * ncols_all[cur[0]] + 16 or len(cur) >= 4096 or grow > self.budget):
+                groups.append(cur)
+                cur = []
+            cur.append(i)
+        if cur:
+            groups.append(cur)
+        cdt = torch.complex128 if self.cplx else torch.float64
+        for g in groups:
+            nmax = int(ncols_all[g].max())
+            lmax = int(lrows[g].max())
+            A, _, _, m_, ncols, keep = self._proxy([cands[i] for i in g], centers[g], length, nproxy, lda=nmax)
+            Ab = A.view(cdt) if self.cplx else A
+            Ab = Ab.view(len(g), m_, nmax)
+            Om = torch.randn(lmax + check_rows, m_, dtype=torch.float64, device=self.device, generator=self.gen)
+            Om = Om.to(cdt) if self.cplx else Om
+            YZ = torch.matmul(Om, Ab)  # (nb, lmax + check_rows, nmax)
+            Y = YZ[:, :lmax].contiguous()
+            Z = YZ[:, lmax:].cpu().numpy()
+            Yr = torch.view_as_real(Y).reshape(-1) if self.cplx else Y.reshape(-1)
+            y_off = torch.as_tensor(np.arange(len(g), dtype=np.int64) * lmax * nmax, device=self.device)
+            res = self._qr_trsm(Yr, y_off, nmax, lrows[g], ncols, lrows[g])
+            for j, (i, r) in enumerate(zip(g, res)):
+                if r is not None and accept_sketch(Z[j, :, :int(ncols_all[i])], r, int(lrows[i]), self.tol,
+                                                cap=min(m_, int(ncols_all[i]))):
+                    results[i] = r
+            del A, Ab, Y, YZ, keep
+        self.timings["sketched"] = self.timings.get("sketched", 0.0) + perf_counter() - t0
+        return results

@@ -31,6 +31,14 @@ EXPORTED_SYMBOLS = (
     "skf_plan_destroy",
     "skf_apply",
     "skf_p2p_blocks",
+    "skf_build_last_error",
+    "skf_proxy_matrices",
+    "skf_pqr_batched",
+    "skf_trsm_batched",
+    "skf_mlp_param_count",
+    "skf_mlp_forward",
+    "skf_mlp_backward",
+    "skf_adam_step",
 )
 
 _vp = ctypes.c_void_p
@@ -89,8 +97,18 @@ def lib():
     L.skf_plan_destroy.argtypes = [_vp]
     L.skf_apply.argtypes = [_vp, _vp, _vp, _vp]
     L.skf_p2p_blocks.argtypes = [_i, _i, _i, ctypes.c_double, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp]
+    L.skf_build_last_error.restype = ctypes.c_char_p
+    L.skf_proxy_matrices.argtypes = [_i, _i, ctypes.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i,
+                                     _vp, _vp, _i, _i, _i, _vp]
+    L.skf_pqr_batched.argtypes = [_vp, _vp, _i, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _vp, _i, _i, _i,
+                                  _i, _vp]
+    L.skf_trsm_batched.argtypes = [_vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]
+    L.skf_mlp_forward.argtypes = [_vp, _vp, _i, _vp, _vp]
+    L.skf_mlp_backward.argtypes = [_vp, _vp, _vp, _i, _vp, _vp, _vp]
+    L.skf_adam_step.argtypes = [_vp, _vp, _vp, _vp, _i, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                ctypes.c_float, _i, _vp]
     for name in EXPORTED_SYMBOLS:
-        if name != "skf_last_error" and name != "skf_version":
+        if name not in ("skf_last_error", "skf_version", "skf_build_last_error"):
             getattr(L, name).restype = _i
     _LIB = L
     return L
@@ -99,6 +117,11 @@ def lib():
 def check(rc):
     if rc != 0:
         raise RuntimeError("skelfmm_hip: " + lib().skf_last_error().decode(errors="replace"))
+
+
+def check_build(rc):
+    if rc != 0:
+        raise RuntimeError("skelfmm_hip (build): " + lib().skf_build_last_error().decode(errors="replace"))
 
 
 def require_gpu():

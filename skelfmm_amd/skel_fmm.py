@@ -5,7 +5,10 @@ skeletons on the host and uploads the box layout to the GPU; ``apply(x)``
 runs the HIP apply and returns a host numpy array. Constructor keywords,
 attributes, error types, the lazy-build notice, ``relerr_fmm_apply_check``
 and ``benchmark_matvec`` follow the reference. Opt-in extension: ``apply``
-also accepts an ``(N, R)`` array of right-hand sides when ``multi_rhs=True``.
+also accepts an ``(N, R)`` array of right-hand sides when ``multi_rhs=True``;
+``build_device="gpu"`` runs the per-box interpolative decompositions as batched
+HIP kernels, and ``rank_model`` (a ``learned_rank.RankMLP`` or ``"default"``)
+switches them to the learned fixed-rank sketched form with adaptive fallback.
 """
 
 from __future__ import annotations
@@ -49,7 +52,8 @@ class SkelFMM:
 
     def __init__(self, points, operator, geometry=None, tol=1e-5, rank_or_tol=None, leaf_size=200,
                  use_single_precision=False, param=0.0, quiet=True, keep_host_fmm=False, defer_build=False,
-                 npoints_max=500, p="auto", max_bs=-1, max_rank=-1, multi_rhs=False):
+                 npoints_max=500, p="auto", max_bs=-1, max_rank=-1, multi_rhs=False, build_device="cpu",
+                 rank_model=None):
         self.points = points
         self.operator = operator
         self.geometry = geometry
@@ -68,6 +72,11 @@ class SkelFMM:
         self._max_bs_limit = max_bs
         self._max_rank_limit = max_rank
         self._multi_rhs = multi_rhs
+        if build_device not in ("cpu", "gpu"):
+            raise ValueError("build_device must be 'cpu' or 'gpu'")
+        self.build_device = build_device
+        self.rank_model = rank_model
+        self.build_stats = {}
         self._is_built = False
         self._lazy_build_notice_printed = False
 
@@ -104,6 +113,10 @@ class SkelFMM:
         r = r[r > 0]
         return int(r.max()) if r.size else 0
 
+    @staticmethod
+    def _synchronize_default():
+        torch.cuda.synchronize()
+
     def _synchronize(self):
         if isinstance(self.device, torch.device) and self.device.type == "cuda":
             torch.cuda.synchronize(self.device)
@@ -131,9 +144,22 @@ class SkelFMM:
             self.fmm = RecursiveSkeletonization(self.tree, self.operator, kappa=self.param, tol=self.rank_or_tol,
                                                 max_bs=self._max_bs_limit, max_rank=self._max_rank_limit,
                                                 geometry=self.geometry)
+            model = self.rank_model
+            if isinstance(model, str) and model == "default":
+                from .learned_rank import load_default
+
+                model = load_default()
+                if model is None:
+                    raise RuntimeError("rank_model='default' but no trained rank network is installed")
+            if model is not None and self.build_device != "gpu":
+                raise ValueError("rank_model requires build_device='gpu'")
             t0 = perf_counter()
-            self.fmm.skel_tree(npoints_max=npoints_max, p=p, verbose=verbose)
+            self.fmm.skel_tree(npoints_max=npoints_max, p=p, verbose=verbose, device=self.build_device,
+                               rank_model=model)
+            if self.build_device == "gpu":
+                self._synchronize_default()
             self.skel_time_seconds = perf_counter() - t0
+            self.build_stats = dict(self.fmm.gpu_stats)
 
             t0 = perf_counter()
             self.fmm.setup_lists(verbose=verbose)
