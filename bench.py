@@ -36,7 +36,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# flops per interaction (FMA = 2): dx,dy,dz (3) + r^2 (mul + 2 fma = 5) + rsqrt (1) + q*K accumulate (2)
+# flops per interaction (FMA = 2): dx,dy,dz (3) + r^2 (mul + 2 fma = 5) + rsqrt (1) + q*K accumulate (2).
+# The symmetric near field evaluates K once for both directions of a pair, so flops per launch are
+# counted as evaluations x (flops - ACC_FLOPS) + interactions x ACC_FLOPS.
+ACC_FLOPS = 2
 FLOPS_PER_INTERACTION = {"laplace_3d": 11, "laplace_2d": 11,
                          # + r = r^2 rsqrt (1), kappa r (1), sin and cos (2), e^{ikr}/r (2), complex q K (8)
                          "helmholtz_3d": 23,
@@ -46,7 +49,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # HBM bytes per launch of the FP32 near-field kernel on the default workload, from the two
 # rocprofv3 --pmc passes summarised in profiles/apply_laplace3d_uniform1M_hbm_traffic.md
 # (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, scale factors from a 512 MiB copy of the same run)
-P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400): 2549.1e6 + 14.2e6}
+P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400, False): 2549.1e6 + 14.2e6}
 
 
 def log(msg):
@@ -464,31 +467,20 @@ def main():
     t32, y32 = timed(d32, q32)
     log(f"apply FP64 {t64 * 1e3:.2f} ms, FP32 {t32 * 1e3:.2f} ms, ratio {t64 / t32:.2f}")
 
-    # near-field kernel alone, HIP events on the current stream
+    # near-field launch alone (the plan's own arguments), HIP events on the stream it runs on
     def p2p_time(devapply, width=1, reps=10):
-        plan = devapply._plan(width)
-        a = devapply._p2p_args(width)
-        a.q = hip.ptr(plan["qbuf"])
-        a.u = hip.ptr(plan["ubuf"])
-        st = hip.stream_handle(dev)
-        L = hip.lib()
-        import ctypes
-
-        hip.check(L.skf_p2p(ctypes.byref(a), st))
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(reps):
-            hip.check(L.skf_p2p(ctypes.byref(a), st))
-        e1.record()
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / reps * 1e-3
+        return devapply.time_near_field(width, reps=reps) * 1e-3
 
     p64 = p2p_time(d64)
     p32 = p2p_time(d32)
     fpi = FLOPS_PER_INTERACTION[op.name]
-    tf64 = d64.interactions * fpi / p64 / 1e12
-    tf32 = d32.interactions * fpi / p32 / 1e12
+
+    def nf_flops(d):
+        ev = d.evaluations if d.use_symmetric else d.interactions
+        return ev * (fpi - ACC_FLOPS) + d.interactions * ACC_FLOPS
+
+    tf64 = nf_flops(d64) / p64 / 1e12
+    tf32 = nf_flops(d32) / p32 / 1e12
     log(f"near-field kernel FP64 {p64 * 1e3:.2f} ms ({tf64:.1f} TF/s), FP32 {p32 * 1e3:.2f} ms ({tf32:.1f} TF/s)")
 
     # errors against the FP64 direct sum at sampled targets
@@ -559,12 +551,14 @@ def main():
             "peak": PEAK_TFLOPS["fp32"],
             "unit": "TFLOP/s",
             "frac": round(tf32 / PEAK_TFLOPS["fp32"], 4),
-            "traffic": P2P_FP32_TRAFFIC_BYTES.get((N, args.tol, args.leaf)),
+            "traffic": P2P_FP32_TRAFFIC_BYTES.get((N, args.tol, args.leaf, bool(d32.use_symmetric))),
             "traffic_unit": "bytes per launch (HBM read + write)",
             "compute_unit": "VALU (per-pair kernel evaluation; no matrix cores at 1 RHS)",
-            "kernel": "p2p_kernel<float, LAPLACE_3D> (near field, VALU/transcendental bound)",
+            "kernel": ("near field: p2p_sym_kernel<float, LAPLACE_3D> (pairs, both directions) + p2p_kernel "
+                       "(self segments)" if d32.use_symmetric else "p2p_kernel<float, LAPLACE_3D> (near field)"),
             "kernel_ms": round(p32 * 1e3, 4),
             "flops_per_interaction": fpi,
+            "kernel_evaluations": int(d32.evaluations if d32.use_symmetric else d32.interactions),
             "fp64_kernel": {"achieved": round(tf64, 3), "peak": PEAK_TFLOPS["fp64"],
                             "frac": round(tf64 / PEAK_TFLOPS["fp64"], 4), "kernel_ms": round(p64 * 1e3, 4)},
         },

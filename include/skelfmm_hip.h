@@ -73,6 +73,31 @@ enum skf_status {
  * kernels `width` = 2*nrhs (interleaved re/im). Real kernels treat complex
  * vectors as 2*nrhs real channels.
  */
+/*
+ * Split near-field work for the symmetric single-layer kernels (Laplace / Helmholtz, 2D and 3D):
+ * the same group / entry / item tables as below, once for the pairs of lists that interact
+ * in both directions ("sym": one representative entry per pair, items of up to 512 targets)
+ * and once for the rest ("one": self segments and unpaired entries, items of up to 64 targets).
+ * A sym entry s of group g adds K(t, s) q[e_q + j] into u[g_out + i] and K(t, s) q[g_out + i]
+ * into u[e_q + j]: the reverse output of an entry is its charge offset (a set's charges and
+ * potentials share slots and kind).
+ */
+typedef struct skf_p2p_lists {
+  int ngroups;
+  const int* g_pt;
+  const int* g_n;
+  const int* g_out;
+  const int* g_box;
+  int nentries;
+  const int* e_pt;
+  const int* e_n;
+  const int* e_q;
+  const int* e_box;
+  const int* e_flags;
+  int nitems;
+  const int* items;
+} skf_p2p_lists;
+
 typedef struct skf_p2p_args {
   int kernel;         /* enum skf_kernel */
   int precision;      /* 0: FP64 points/vectors, 1: FP32 (box-centred points, FP64 centres) */
@@ -102,6 +127,14 @@ typedef struct skf_p2p_args {
   const int* items;   /* int4 per item */
   int mfma_min_channels; /* real kernels (FP32 or FP64) with at least this many channels use the
                             matrix-core kernel; 0 = default (16), negative = never */
+  /* optional symmetric split (one channel, symmetric kernel): used instead of the lists above
+     when both are set; the FP32 form needs `urev`, an FP64 buffer of urev_len scalars covering
+     every u index (2 * nslots * width in the apply), zeroed and folded into u by the call */
+  const skf_p2p_lists* one;
+  const skf_p2p_lists* sym;
+  double* urev;
+  long long urev_len;
+  int sym_unroll;              /* sources per inner step of the symmetric kernel: 1, 2, 4 (0 = 2) */
 } skf_p2p_args;
 
 /* one level of the upward or downward interpolation pass */
@@ -193,17 +226,26 @@ int skf_proxy_matrices(int kernel, int ndim, double kappa, const double* pts, co
                        const double* unit_proxy, int nproxy, int stacked, int complex_out, void* A,
                        const long long* a_off, int lda, int nbox, int max_cols, void* stream);
 
-/* In-place column-pivoted Householder QR of every A_b for at most kmax[b] steps, stopping at the
+/* In-place column-pivoted Householder QR of every A_b (row stride lda[b], or n[b] when lda is NULL)
+   for at most kmax[b] steps, stopping at the
    first |R_ii| <= tol |R_00|: rank[b] = that i (converged[b] = 1), or kmax[b] with converged[b] = 0
    when the cutoff was not reached within kmax[b] < min(m, n) steps. perm (at p_off[b]) holds the
    column pivots, rdiag the |R_ii|. */
-int skf_pqr_batched(void* A, const long long* a_off, int lda, const int* m, const int* n, const int* kmax,
+int skf_pqr_batched(void* A, const long long* a_off, const int* lda, const int* m, const int* n, const int* kmax,
                     double tol, int* rank, int* converged, int* perm, const long long* p_off, double* rdiag, int nbox,
                     int max_m, int max_n, int complex_a, void* stream);
 
 /* T_b = R11^{-1} R12 (rank x (n - rank), row-major at t_off[b]) from the factored A_b. */
-int skf_trsm_batched(const void* A, const long long* a_off, int lda, const int* n, const int* rank, void* T,
+int skf_trsm_batched(const void* A, const long long* a_off, const int* lda, const int* n, const int* rank, void* T,
                      const long long* t_off, int nbox, int max_cols, int complex_a, void* stream);
+
+/* Proxy-residual check of the IDs (perm, rank, T as produced above) on sketches Z_b (c rows,
+   row stride ldz[b] or n[b] when ldz is NULL, at z_off[b]) of the proxy matrices: res[b] = max_j |Z_red,j - Z_skel T_j|_2,
+   znorm[b] = max_j |Z_j|_2. Used by the learned-rank build to accept or reject a sketched ID;
+   the reference always factors the whole proxy matrix (util.py:13-51). */
+int skf_id_residual(const void* Z, const long long* z_off, const int* ldz, int c, const int* n, const int* rank,
+                    const int* perm, const long long* p_off, const void* T, const long long* t_off, int nbox,
+                    int complex_a, double* res, double* znorm, void* stream);
 
 /* ------------------------------------------------- learned rank predictor
  * No counterpart in the reference (which always runs the full adaptive ID):

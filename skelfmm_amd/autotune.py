@@ -8,7 +8,10 @@ near-field launch over
   workgroups -- trades load balance against atomics and re-staging), and
 * for FP32 real kernels with at least 16 channels, the matrix-core kernel
   against the VALU kernel (RHS blocking: 64 channels per MFMA block, or the
-  widest VALU channel block that fits LDS),
+  widest VALU channel block that fits LDS), and
+* where the symmetric near-field kernel runs (one channel, single-layer kernel),
+  the number of sources per step of its inner loop (1, 2 or 4: instruction-level
+  parallelism against registers, i.e. waves per SIMD),
 
 keeps the fastest and caches the choice per (device name, operator, precision,
 width, slots, interactions) in a JSON file (``$SKELFMM_AMD_TUNE_CACHE``, default
@@ -57,6 +60,8 @@ def _store(path, db):
 
 def apply_setting(dev, setting):
     dev.set_mfma_min_channels(int(setting.get("mfma_min_channels", 0)))
+    if hasattr(dev, "set_sym_unroll"):
+        dev.set_sym_unroll(int(setting.get("sym_unroll", 0)))
     if int(setting["max_sources"]) != int(getattr(dev.layout, "max_sources", -1)):
         dev.set_near_field_split(int(setting["max_sources"]))
 
@@ -73,16 +78,21 @@ def tune(dev, width=1, reps=3, use_cache=True, splits=SPLITS):
     mfma_opts = [0]
     if dev.precision == 1 and not dev.operator.is_complex and nch >= 16:
         mfma_opts = [0, -1]
+    sym = getattr(dev, "use_symmetric", False) and nch == 1
+    unrolls = (1, 2, 4) if sym else (0,)
     results = []
     for mf in mfma_opts:
         dev.set_mfma_min_channels(mf)
-        for ms in splits:
-            dev.set_near_field_split(ms)
-            t = dev.time_near_field(width, reps=reps)
-            results.append((t, ms, mf))
-    t, ms, mf = min(results)
-    best = {"max_sources": ms, "mfma_min_channels": mf, "near_field_ms": round(t, 4),
-            "tried": [[round(a, 4), b, c] for a, b, c in results]}
+        for un in unrolls:
+            if hasattr(dev, "set_sym_unroll"):
+                dev.set_sym_unroll(un)
+            for ms in splits:
+                dev.set_near_field_split(ms)
+                t = dev.time_near_field(width, reps=reps)
+                results.append((t, ms, mf, un))
+    t, ms, mf, un = min(results)
+    best = {"max_sources": ms, "mfma_min_channels": mf, "sym_unroll": un, "near_field_ms": round(t, 4),
+            "tried": [[round(a, 4), b, c, d] for a, b, c, d in results]}
     apply_setting(dev, best)
     if use_cache:
         db[key] = best

@@ -136,7 +136,21 @@ class SkelFMMBatched(torch.nn.Module):
         self._nf = {k: i32(getattr(L, k)) for k in
                     ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags")}
         self._nf["items"] = i32(L.items.reshape(-1))
+        # symmetric split of the near field (single-layer kernels): pair lists + the rest
+        self._split = None
+        if getattr(L, "split", None) is not None:
+            self._split = []
+            for part in L.split:
+                t = {k: i32(getattr(part, k)) for k in
+                     ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags")}
+                t["items"] = i32(part.items.reshape(-1))
+                self._split.append(t)
+            self._split_structs = [hip.P2PLists(), hip.P2PLists()]
+            self._fill_split_structs()
+        self.use_symmetric = self._split is not None
+        self.sym_unroll = 0
         self.interactions = L.interactions
+        self.evaluations = getattr(L, "evaluations", L.interactions)
         self.nslots = L.nslots
         self.N = L.N
         for k in list(self._plans):
@@ -144,7 +158,29 @@ class SkelFMMBatched(torch.nn.Module):
         self._graphs = {}
         return self
 
-    def _p2p_args(self, width):
+    def _fill_split_structs(self):
+        for st, t in zip(self._split_structs, self._split):
+            st.ngroups = int(t["g_pt"].shape[0])
+            st.nentries = int(t["e_pt"].shape[0])
+            st.nitems = int(t["items"].shape[0] // 4)
+            for k in ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags", "items"):
+                setattr(st, k, hip.ptr(t[k]))
+
+    def set_sym_unroll(self, n):
+        """Sources per inner step of the symmetric near-field kernel (1, 2 or 4; 0 = default 2)."""
+        self.sym_unroll = int(n)
+        for k in list(self._plans):
+            self._destroy_plan(k)
+        self._graphs = {}
+
+    def set_symmetric(self, on):
+        """Use (True, default when available) or bypass the symmetric near-field kernel."""
+        self.use_symmetric = bool(on) and self._split is not None
+        for k in list(self._plans):
+            self._destroy_plan(k)
+        self._graphs = {}
+
+    def _p2p_args(self, width, urev=None):
         a = hip.P2PArgs()
         a.kernel = self.operator.kernel_id
         a.precision = self.precision
@@ -166,7 +202,21 @@ class SkelFMMBatched(torch.nn.Module):
         a.nentries = int(nf["e_pt"].shape[0])
         a.nitems = int(nf["items"].shape[0] // 4)
         a.mfma_min_channels = int(self.mfma_min_channels)
+        a.sym_unroll = int(self.sym_unroll)
+        if self.use_symmetric and (self.precision == 0 or urev is not None):
+            a.one = ctypes.pointer(self._split_structs[0])
+            a.sym = ctypes.pointer(self._split_structs[1])
+            if urev is not None:
+                a.urev = hip.ptr(urev)
+                a.urev_len = int(urev.numel())
         return a
+
+    def _urev(self, width):
+        """FP64 buffer for the symmetric kernel's reverse sums (FP32 form, one channel)."""
+        cx = self.operator.is_complex or self.kappa > 0
+        if not self.use_symmetric or self.precision != 1 or width != (2 if cx else 1):
+            return None
+        return torch.zeros(2 * self.nslots * width, dtype=torch.float64, device=self.device)
 
     def _plan(self, width):
         p = self._plans.get(width)
@@ -190,10 +240,11 @@ class SkelFMMBatched(torch.nn.Module):
         d.up = ctypes.cast(self._up_arr, ctypes.POINTER(hip.LevelArgs))
         d.ndown = self._ndown
         d.down = ctypes.cast(self._down_arr, ctypes.POINTER(hip.LevelArgs))
-        d.p2p = self._p2p_args(width)
+        urev = self._urev(width)
+        d.p2p = self._p2p_args(width, urev)
         handle = ctypes.c_void_p()
         hip.check(hip.lib().skf_plan_create(ctypes.byref(d), ctypes.byref(handle)))
-        p = {"handle": handle, "qbuf": qbuf, "ubuf": ubuf, "desc": d}
+        p = {"handle": handle, "qbuf": qbuf, "ubuf": ubuf, "urev": urev, "desc": d}
         self._plans[width] = p
         return p
 
@@ -248,6 +299,11 @@ class SkelFMMBatched(torch.nn.Module):
         resplit(self.layout, max_sources)
         self._nf["items"] = torch.as_tensor(np.ascontiguousarray(self.layout.items.reshape(-1), dtype=np.int32),
                                             device=self.device)
+        if self._split is not None:
+            for t, part in zip(self._split, self.layout.split):
+                t["items"] = torch.as_tensor(np.ascontiguousarray(part.items.reshape(-1), dtype=np.int32),
+                                             device=self.device)
+            self._fill_split_structs()
         for k in list(self._plans):
             self._destroy_plan(k)
         self._graphs = {}
@@ -255,7 +311,7 @@ class SkelFMMBatched(torch.nn.Module):
     def time_near_field(self, width, reps=5):
         """Mean duration (ms) of one near-field launch for ``width`` channels, HIP events on the current stream."""
         p = self._plan(width)
-        a = self._p2p_args(width)
+        a = self._p2p_args(width, p["urev"])
         a.q = hip.ptr(p["qbuf"])
         a.u = hip.ptr(p["ubuf"])
         st = hip.stream_handle(self.device)
@@ -304,6 +360,9 @@ class SkelFMMBatched(torch.nn.Module):
         acc = self.XX_slots.numel() * self.XX_slots.element_size()
         for t in self._nf.values():
             acc += t.numel() * t.element_size()
+        for part in self._split or []:
+            for t in part.values():
+                acc += t.numel() * t.element_size()
         for st in self._stages:
             for t in st.values():
                 acc += t.numel() * t.element_size()

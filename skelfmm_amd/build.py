@@ -219,12 +219,13 @@ class RecursiveSkeletonization:
 
             F = box_features(self, lev, boxes, centers, length)
             k_pred = rank_model.predict_rank(F)
-            res = gpu.sketched(cands, centers, length, nproxy, k_pred)
-            self.gpu_stats["sketched"] += sum(r is not None for r in res)
+            res = gpu.sketched(cands, centers, length, nproxy, k_pred, **self._sketch_opts)
+            self.gpu_stats["sketched"] += sum(r is not None and not isinstance(r, str) for r in res)
+            self.gpu_stats["direct"] += sum(isinstance(r, str) for r in res)
+            self.gpu_stats["fallback"] += sum(r is None for r in res)
+            res = [None if isinstance(r, str) else r for r in res]
         todo = [i for i, r in enumerate(res) if r is None]
         if todo:
-            if rank_model is not None:
-                self.gpu_stats["fallback"] += len(todo)
             sub = gpu.adaptive([cands[i] for i in todo], centers[todo], length, nproxy)
             for i, r in zip(todo, sub):
                 res[i] = r
@@ -244,24 +245,30 @@ class RecursiveSkeletonization:
         self.proj[box] = proj
 
     def skel_tree(self, npoints_max=500, p="auto", verbose=False, min_parallel_items=64, nworkers=None,
-                  device="cpu", rank_model=None):
+                  device="cpu", rank_model=None, gpu_options=None):
         """Bottom-up skeletonization until level 2 or ``sum(bs) <= npoints_max`` (``recursive_skel.py:326-502``).
 
         ``device="gpu"`` runs the per-box IDs of a level as batched GPU kernels
         (``skelfmm_amd.gpu_build``): the adaptive pivoted QR, or -- with ``rank_model``
         (a trained ``learned_rank.RankMLP``) -- the sketched fixed-rank QR with the
-        proxy-residual check, falling back to the adaptive QR box by box.
+        proxy-residual check, falling back to the adaptive QR box by box. ``gpu_options``:
+        ``{"profile": bool, "sketch": {keyword arguments of GPUSkeletonizer.sketched}}``.
         """
         if nworkers is None:
             nworkers = _default_workers()
         self.root_level = max(0, self.tree.nlevels - 1)
         self.build_device = device
-        self.gpu_stats = {"sketched": 0, "fallback": 0, "adaptive": 0, "cpu": 0}
+        # boxes by path: accepted sketches, sketches rejected (fallback), predicted incompressible
+        # (direct), adaptive GPU IDs (fallback + direct + no model), host IDs (LDS too small)
+        self.gpu_stats = {"sketched": 0, "fallback": 0, "direct": 0, "adaptive": 0, "cpu": 0}
         gpu = None
         if device == "gpu":
             from .gpu_build import GPUSkeletonizer
 
             gpu = GPUSkeletonizer(self.operator, self.kappa, self.tol, self.tree.XX, self.geometry)
+            gpu.profile = bool((gpu_options or {}).get("profile", False))
+        self.gpu_skel = gpu
+        self._sketch_opts = dict((gpu_options or {}).get("sketch", {}))
         pool = None
         try:
             for lev in range(self.tree.nlevels - 1, 0, -1):

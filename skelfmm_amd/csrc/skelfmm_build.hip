@@ -7,6 +7,7 @@
 //   skf_pqr_batched     column-pivoted Householder QR of every A_b in place, stopped at the first
 //                       |R_ii| <= tol |R_00| (adaptive rank, util.py:25-31) or after kmax_b steps
 //   skf_trsm_batched    T_b = R11^{-1} R12 (util.py:43-49)
+//   skf_id_residual     proxy-residual check of an ID on an independent sketch (learned-rank build)
 // Learned rank predictor (8 -> 64 -> 64 -> 1 MLP, GELU, FP32):
 //   skf_mlp_forward / skf_mlp_backward / skf_adam_step
 #include <hip/hip_runtime.h>
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(kT) void pqr_kernel(S* __restrict__ A, const long l
                                                  const int* __restrict__ kmax, double tol, int* __restrict__ rank_out,
                                                  int* __restrict__ conv_out, int* __restrict__ perm,
                                                  const long long* __restrict__ p_off, double* __restrict__ rdiag,
-                                                 int lda0) {
+                                                 const int* __restrict__ lda_b) {
   using C = Sc<S>;
   extern __shared__ double sh[];
   __shared__ double red[4];
@@ -179,7 +180,7 @@ __global__ __launch_bounds__(kT) void pqr_kernel(S* __restrict__ A, const long l
   const int b = blockIdx.x;
   const int m = mm[b], n = nn[b];
   const int kcap = min(kmax[b], min(m, n));
-  const long long n_ = lda0 > 0 ? lda0 : n;  // row stride
+  const long long n_ = lda_b ? lda_b[b] : n;  // row stride
   S* a = A + a_off[b];
   int* pv = perm + p_off[b];
   double* rd = rdiag + p_off[b];
@@ -300,20 +301,72 @@ __global__ __launch_bounds__(kT) void pqr_kernel(S* __restrict__ A, const long l
 template <typename S>
 __global__ __launch_bounds__(kT) void trsm_kernel(const S* __restrict__ A, const long long* __restrict__ a_off,
                                                   const int* __restrict__ nn, const int* __restrict__ rank,
-                                                  S* __restrict__ T, const long long* __restrict__ t_off, int lda0) {
+                                                  S* __restrict__ T, const long long* __restrict__ t_off,
+                                                  const int* __restrict__ lda_b) {
   using C = Sc<S>;
   const int b = blockIdx.y;
   const int n = nn[b], r = rank[b];
   const int nc = n - r;
   const int c = blockIdx.x * kT + threadIdx.x;
   if (r <= 0 || c >= nc) return;
-  const long long lda = lda0 > 0 ? lda0 : n;
+  const long long lda = lda_b ? lda_b[b] : n;
   const S* a = A + a_off[b];
   S* t = T + t_off[b];
   for (int i = r - 1; i >= 0; --i) {
     S s = a[i * lda + r + c];
     for (int l = i + 1; l < r; ++l) s = s - a[i * lda + l] * t[(long long)l * nc + c];
     t[(long long)i * nc + c] = C::div(s, a[i * lda + i]);
+  }
+}
+
+// Proxy-residual check of an ID from an independent sketch Z_b (c rows, row stride ldz):
+// res[b] = max_j |Z[:, p[r + j]] - sum_i Z[:, p[i]] T[i][j]|_2 and znorm[b] = max_j |Z[:, j]|_2.
+// One workgroup per box, threads over the redundant columns; block max through LDS.
+template <typename S>
+__global__ __launch_bounds__(kT) void id_residual_kernel(const S* __restrict__ Z, const long long* __restrict__ z_off,
+                                                         const int* __restrict__ ldz_b, int c, const int* __restrict__ nn,
+                                                         const int* __restrict__ rank, const int* __restrict__ perm,
+                                                         const long long* __restrict__ p_off, const S* __restrict__ T,
+                                                         const long long* __restrict__ t_off,
+                                                         double* __restrict__ res, double* __restrict__ znorm) {
+  using C = Sc<S>;
+  __shared__ double red[2][4];
+  const int b = blockIdx.x;
+  const int n = nn[b], r = rank[b];
+  const int nc = n - r;
+  const S* z = Z + z_off[b];
+  const long long ldz = ldz_b ? ldz_b[b] : n;
+  const int* pv = perm + p_off[b];
+  const S* t = T + t_off[b];
+  double emax = 0.0, zmax = 0.0;
+  for (int j = threadIdx.x; j < n; j += kT) {
+    double s = 0.0;
+    for (int row = 0; row < c; ++row) s += C::abs2(z[(long long)row * ldz + j]);
+    zmax = fmax(zmax, s);
+  }
+  for (int j = threadIdx.x; j < nc; j += kT) {
+    const int cj = pv[r + j];
+    double s = 0.0;
+    for (int row = 0; row < c; ++row) {
+      const S* zr = z + (long long)row * ldz;
+      S e = zr[cj];
+      for (int i = 0; i < r; ++i) e = e - zr[pv[i]] * t[(long long)i * nc + j];
+      s += C::abs2(e);
+    }
+    emax = fmax(emax, s);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    emax = fmax(emax, __shfl_xor(emax, o, 64));
+    zmax = fmax(zmax, __shfl_xor(zmax, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = emax;
+    red[1][threadIdx.x >> 6] = zmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    res[b] = sqrt(fmax(fmax(red[0][0], red[0][1]), fmax(red[0][2], red[0][3])));
+    znorm[b] = sqrt(fmax(fmax(red[1][0], red[1][1]), fmax(red[1][2], red[1][3])));
   }
 }
 
@@ -476,7 +529,7 @@ extern "C" int skf_proxy_matrices(int kernel, int ndim, double kappa, const doub
   return SKF_OK;
 }
 
-extern "C" int skf_pqr_batched(void* A, const long long* a_off, int lda, const int* m, const int* n,
+extern "C" int skf_pqr_batched(void* A, const long long* a_off, const int* lda, const int* m, const int* n,
                                const int* kmax, double tol, int* rank, int* converged, int* perm,
                                const long long* p_off, double* rdiag, int nbox, int max_m, int max_n, int complex_a,
                                void* stream) {
@@ -501,7 +554,7 @@ extern "C" int skf_pqr_batched(void* A, const long long* a_off, int lda, const i
   return SKF_OK;
 }
 
-extern "C" int skf_trsm_batched(const void* A, const long long* a_off, int lda, const int* n, const int* rank,
+extern "C" int skf_trsm_batched(const void* A, const long long* a_off, const int* lda, const int* n, const int* rank,
                                 void* T, const long long* t_off, int nbox, int max_cols, int complex_a,
                                 void* stream) {
   if (nbox <= 0) return SKF_OK;
@@ -514,6 +567,23 @@ extern "C" int skf_trsm_batched(const void* A, const long long* a_off, int lda, 
   else
     hipLaunchKernelGGL((trsm_kernel<double>), grid, dim3(kT), 0, st, static_cast<const double*>(A), a_off, n, rank,
                        static_cast<double*>(T), t_off, lda);
+  BLD_LAUNCHED();
+  return SKF_OK;
+}
+
+extern "C" int skf_id_residual(const void* Z, const long long* z_off, const int* ldz, int c, const int* n, const int* rank,
+                               const int* perm, const long long* p_off, const void* T, const long long* t_off,
+                               int nbox, int complex_a, double* res, double* znorm, void* stream) {
+  if (nbox <= 0) return SKF_OK;
+  if (!Z || !z_off || !n || !rank || !perm || !p_off || !T || !t_off || !res || !znorm || c <= 0)
+    return berr(SKF_ERR_ARG, "bad ID-residual arguments");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (complex_a)
+    hipLaunchKernelGGL((id_residual_kernel<zd>), dim3(nbox), dim3(kT), 0, st, static_cast<const zd*>(Z), z_off, ldz, c,
+                       n, rank, perm, p_off, static_cast<const zd*>(T), t_off, res, znorm);
+  else
+    hipLaunchKernelGGL((id_residual_kernel<double>), dim3(nbox), dim3(kT), 0, st, static_cast<const double*>(Z), z_off,
+                       ldz, c, n, rank, perm, p_off, static_cast<const double*>(T), t_off, res, znorm);
   BLD_LAUNCHED();
   return SKF_OK;
 }

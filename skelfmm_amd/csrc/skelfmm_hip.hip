@@ -187,6 +187,7 @@ struct P2PDev {
   int width;
   int nch;  // channels (complex channels for complex kernels)
   double kappa;
+  double* urev;  // reverse outputs of the symmetric kernel (FP64; u itself in the FP64 form)
 };
 
 __device__ inline void atomic_acc(double* p, double v) { unsafeAtomicAdd(p, v); }
@@ -351,6 +352,206 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
       }
     }
   }
+}
+
+// ------------------------------------------- symmetric near field (single-layer kernels)
+// For two point sets U, V that interact in both directions with the same sign and a symmetric
+// kernel (K(x, y) = K(y, x)), one evaluation of K(t, s) serves u_U[t] += K q_V[s] and
+// u_V[s] += K q_U[t]. A work item holds up to kSymTargets targets of one group, taken in passes
+// of 64 * TPL (TPL per lane, in registers, with their own charges q[g_out + t]), and a run of
+// its source entries; the
+// 4 waves split every 256-source tile. Forward sums stay in the lane (FP64 across tiles, combined
+// over the waves through LDS at the end). Reverse sums: every lane writes its partial for each
+// source of a block of SUB sources to a per-wave LDS table; the wave then sums the table's
+// columns (64 / SUB lanes per source, a butterfly over those lanes) and adds each source's total
+// with an FP64 atomic at urev[e_q + s].
+constexpr int kSymTargets = 512;  // targets per work item (the host's SYM_TARGETS_PER_ITEM)
+
+// targets per lane and pass: 256-target passes fit the typical leaf (a few hundred points)
+// without idle lanes and keep the registers at four waves per SIMD or more
+template <typename R>
+struct SymTPL {
+  static constexpr int n = 4;
+};
+
+// sources per reverse block: the per-wave table (n x 64 partials) stays at 4 KiB
+template <typename R, int CW>
+struct SymSub {
+  static constexpr int n = 16 * 4 / int(sizeof(R) * CW);
+  static constexpr int lanes = 64 / n;  // lanes summing one source's column
+};
+
+// Orders one wave's LDS accesses around its reverse table: LDS instructions of a wave execute
+// in order, so only the compiler must not move accesses across this point (no wait on the
+// global atomics in flight).
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
+template <typename R, int OP, bool PREC, int UNR>
+__global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
+  using S = Stage<OP, 1>;
+  constexpr int NV = S::nv;
+  constexpr int CW = S::cw;
+  constexpr int TPL = SymTPL<R>::n;
+  constexpr int SUB = SymSub<R, CW>::n;
+  constexpr int LPS = SymSub<R, CW>::lanes;
+  static_assert(OpT<OP>::geom == 0, "symmetric evaluation needs a single-layer kernel");
+  static_assert(kSymTargets % (64 * TPL) == 0, "passes must tile a work item");
+  __shared__ R tile[kWG * NV];
+  __shared__ R rvl[4][SUB][CW][64];   // reverse partials of one block of sources, per wave
+  __shared__ double red[4][kTCH][CW];
+
+  const int4 it = d.items[blockIdx.x];
+  const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ng = d.g_n[g];
+  const int gpt = d.g_pt[g], gout = d.g_out[g];
+  const R* px = static_cast<const R*>(d.px);
+  const R* py = static_cast<const R*>(d.py);
+  const R* pz = static_cast<const R*>(d.pz);
+  const R* qv = static_cast<const R*>(d.q);
+  R* uo = static_cast<R*>(d.u);
+  const int w = d.width;
+  double ctx = 0, cty = 0, ctz = 0;
+  if (PREC) {
+    const int gb = d.g_box[g];
+    ctx = d.centers[3 * gb + 0];
+    cty = d.centers[3 * gb + 1];
+    ctz = d.centers[3 * gb + 2];
+  }
+  const R kappa = R(d.kappa);
+  const int tend = min(ng, t0 + kSymTargets);
+
+  for (int tp = t0; tp < tend; tp += 64 * TPL) {  // wave-uniform passes over the item's targets
+    // targets tp + lane + 64 k; padding targets repeat the first one with zero charge
+    R qt[TPL * CW], xt[TPL], yt[TPL], zt[TPL];
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+      const int t = tp + lane + 64 * k;
+      const int sl = gpt + (t < tend ? t : t0);
+      xt[k] = px[sl];
+      yt[k] = py[sl];
+      zt[k] = (S::dim == 3) ? pz[sl] : R(0);
+#pragma unroll
+      for (int c = 0; c < CW; ++c) qt[k * CW + c] = (t < tend) ? qv[(long long)(gout + t) * w + c] : R(0);
+    }
+    double acc[TPL * CW];
+#pragma unroll
+    for (int a = 0; a < TPL * CW; ++a) acc[a] = 0.0;
+
+    for (int s = sb; s < se; ++s) {
+      const int ps = d.e_pt[s], ns = d.e_n[s], qo = d.e_q[s], fl = d.e_flags[s];
+      const double sgn = (fl & SKF_SRC_NEG) ? -1.0 : 1.0;
+      // precision form: sources are shifted into the target box's frame (minus the difference of
+      // the centres), one shift per source for all of a lane's targets
+      R ox = 0, oy = 0, oz = 0;
+      if (PREC) {
+        const int eb = d.e_box[s];
+        ox = R(ctx - d.centers[3 * eb + 0]);
+        oy = R(cty - d.centers[3 * eb + 1]);
+        oz = R(ctz - d.centers[3 * eb + 2]);
+      }
+      for (int base = 0; base < ns; base += kWG) {
+        {
+          const int j = base + tid;
+          R* row = &tile[tid * NV];
+          if (j < ns) {
+            const int sl = ps + j;
+            row[0] = px[sl];
+            row[1] = py[sl];
+            if (S::dim == 3) row[2] = pz[sl];
+            const R* qs = qv + (long long)(qo + j) * w;
+#pragma unroll
+            for (int c = 0; c < CW; ++c) row[S::dim + c] = qs[c];
+          } else {
+#pragma unroll
+            for (int c = 0; c < NV; ++c) row[c] = R(0);
+          }
+        }
+        __syncthreads();
+        const int wb = base + wave * 64;  // wave-uniform
+        const int cnt = __builtin_amdgcn_readfirstlane(max(0, min(64, ns - wb)));
+        const R* tw = &tile[wave * 64 * NV];
+        R part[TPL * CW];
+#pragma unroll
+        for (int a = 0; a < TPL * CW; ++a) part[a] = 0;
+        for (int sub = 0; sub < cnt; sub += SUB) {
+          // rows past the segment end are zero (no charge): blocks are evaluated whole
+#pragma unroll UNR
+          for (int jj = 0; jj < SUB; ++jj) {
+            const R* row = tw + (sub + jj) * NV;
+            const R xs = PREC ? row[0] - ox : row[0];
+            const R ys = PREC ? row[1] - oy : row[1];
+            const R zs = (S::dim == 3) ? (PREC ? row[2] - oz : row[2]) : R(0);
+            R r0 = 0, r1 = 0;
+#pragma unroll
+            for (int k = 0; k < TPL; ++k) {
+              const R dx = xt[k] - xs;
+              const R dy = yt[k] - ys;
+              const R dz = (S::dim == 3) ? zt[k] - zs : R(0);
+              const Cplx<R> kv = kernel_value<R, OP>(dx, dy, dz, R(0), kappa);
+              if (CW == 1) {
+                part[k] += kv.re * row[S::dim];
+                r0 += kv.re * qt[k];
+              } else {
+                const R sr = row[S::dim], si = row[S::dim + 1];
+                part[2 * k] += kv.re * sr - kv.im * si;
+                part[2 * k + 1] += kv.re * si + kv.im * sr;
+                const R tr = qt[2 * k], ti = qt[2 * k + 1];
+                r0 += kv.re * tr - kv.im * ti;
+                r1 += kv.re * ti + kv.im * tr;
+              }
+            }
+            rvl[wave][jj][0][lane] = r0;
+            if (CW == 2) rvl[wave][jj][CW - 1][lane] = r1;
+          }
+          wave_sync_lds();
+          // column sums: lanes j * LPS .. j * LPS + LPS - 1 hold source j's partial over 64 / LPS lanes
+          const int j = lane / LPS, pp = lane % LPS;
+          constexpr int PER = 64 / LPS;
+#pragma unroll
+          for (int c = 0; c < CW; ++c) {
+            R v = 0;
+#pragma unroll
+            for (int i = 0; i < PER; ++i) v += rvl[wave][j][c][pp * PER + i];
+#pragma unroll
+            for (int o = 1; o < LPS; o <<= 1) v += __shfl_xor(v, o, 64);
+            if (pp == 0 && sub + j < cnt)
+              unsafeAtomicAdd(&d.urev[(long long)(qo + wb + sub + j) * w + c], sgn * double(v));
+          }
+          wave_sync_lds();
+        }
+#pragma unroll
+        for (int a = 0; a < TPL * CW; ++a) acc[a] += sgn * double(part[a]);
+        __syncthreads();
+      }
+    }
+    // forward sums: combine the 4 waves, one target slot at a time
+#pragma unroll
+    for (int k = 0; k < TPL; ++k) {
+#pragma unroll
+      for (int c = 0; c < CW; ++c) red[wave][lane][c] = acc[k * CW + c];
+      __syncthreads();
+      const int t = tp + lane + 64 * k;
+      if (wave == 0 && t < tend) {
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+          const double v = red[0][lane][c] + red[1][lane][c] + red[2][lane][c] + red[3][lane][c];
+          atomic_acc(&uo[(long long)(gout + t) * w + c], R(v));
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <typename R>
+__global__ void fold_rev_kernel(R* __restrict__ u, const double* __restrict__ urev, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) u[i] += R(urev[i]);
 }
 
 // Multi-RHS near field on the matrix cores (real kernels, FP32 or FP64): for every 4 sources
@@ -580,6 +781,56 @@ int launch_p2p_prec(int op, const P2PDev& d, int nitems, hipStream_t st, int mfm
   return launch_p2p_op<R, PREC, 16>(op, d, nitems, st);
 }
 
+template <typename R, bool PREC, int UNR>
+int launch_p2p_sym_u(int op, const P2PDev& d, int nitems, hipStream_t st) {
+  const dim3 grid(nitems), block(kWG);
+  switch (op) {
+    case SKF_LAPLACE_2D:
+      hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_LAPLACE_2D, PREC, UNR>), grid, block, 0, st, d);
+      break;
+    case SKF_LAPLACE_3D:
+      hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_LAPLACE_3D, PREC, UNR>), grid, block, 0, st, d);
+      break;
+    case SKF_HELMHOLTZ_2D:
+      hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_HELMHOLTZ_2D, PREC, UNR>), grid, block, 0, st, d);
+      break;
+    case SKF_HELMHOLTZ_3D:
+      hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_HELMHOLTZ_3D, PREC, UNR>), grid, block, 0, st, d);
+      break;
+    default: return set_err(SKF_ERR_ARG, "kernel %d has no symmetric form", op);
+  }
+  SKF_LAUNCHED();
+  return SKF_OK;
+}
+
+// unroll: sources evaluated per step of the symmetric kernel's inner loop (1, 2 or 4; 0 = 2)
+template <typename R, bool PREC>
+int launch_p2p_sym(int op, const P2PDev& d, int nitems, hipStream_t st, int unroll) {
+  if (nitems <= 0) return SKF_OK;
+  if (unroll == 1) return launch_p2p_sym_u<R, PREC, 1>(op, d, nitems, st);
+  if (unroll == 4) return launch_p2p_sym_u<R, PREC, 4>(op, d, nitems, st);
+  if (unroll == 0 || unroll == 2) return launch_p2p_sym_u<R, PREC, 2>(op, d, nitems, st);
+  return set_err(SKF_ERR_ARG, "symmetric unroll must be 0, 1, 2 or 4");
+}
+
+bool op_symmetric(int op) {
+  return op == SKF_LAPLACE_2D || op == SKF_LAPLACE_3D || op == SKF_HELMHOLTZ_2D || op == SKF_HELMHOLTZ_3D;
+}
+
+int check_lists(const skf_p2p_lists* L) {
+  if (L->ngroups < 0 || L->nentries < 0 || L->nitems < 0) return set_err(SKF_ERR_ARG, "bad split list sizes");
+  if (L->nitems > 0 && (!L->items || !L->g_pt || !L->g_n || !L->g_out || !L->g_box || !L->e_pt || !L->e_n ||
+                        !L->e_q || !L->e_box || !L->e_flags))
+    return set_err(SKF_ERR_ARG, "missing split list arrays");
+  return SKF_OK;
+}
+
+void use_lists(P2PDev& d, const skf_p2p_lists* L) {
+  d.g_pt = L->g_pt; d.g_n = L->g_n; d.g_out = L->g_out; d.g_box = L->g_box;
+  d.e_pt = L->e_pt; d.e_n = L->e_n; d.e_q = L->e_q; d.e_box = L->e_box; d.e_flags = L->e_flags;
+  d.items = reinterpret_cast<const int4*>(L->items);
+}
+
 bool op_is_complex(int op) {
   return op == SKF_HELMHOLTZ_2D || op == SKF_HELMHOLTZ_3D || op == SKF_HELMHOLTZ_DL_3D ||
          op == SKF_HELMHOLTZ_ADL_3D;
@@ -606,10 +857,37 @@ int run_p2p(const skf_p2p_args* a, const void* q, void* u, hipStream_t st) {
   d.width = a->width;
   d.nch = cx ? a->width / 2 : a->width;
   d.kappa = a->kappa;
+  d.urev = nullptr;
   const int mfma_min = a->mfma_min_channels == 0 ? 16 : a->mfma_min_channels;
-  if (a->precision == 0) return launch_p2p_prec<double, false>(a->kernel, d, a->nitems, st, mfma_min);
-  if (a->precision == 1) return launch_p2p_prec<float, true>(a->kernel, d, a->nitems, st, mfma_min);
-  return set_err(SKF_ERR_ARG, "precision must be 0 or 1");
+  if (a->precision != 0 && a->precision != 1) return set_err(SKF_ERR_ARG, "precision must be 0 or 1");
+  const bool mfma = !cx && mfma_min > 0 && d.nch >= mfma_min;
+  const bool sym = a->one && a->sym && d.nch == 1 && op_symmetric(a->kernel) && !mfma &&
+                   (a->precision == 0 || (a->urev && a->urev_len > 0));
+  if (!sym) {
+    if (a->precision == 0) return launch_p2p_prec<double, false>(a->kernel, d, a->nitems, st, mfma_min);
+    return launch_p2p_prec<float, true>(a->kernel, d, a->nitems, st, mfma_min);
+  }
+  int rc;
+  if ((rc = check_lists(a->one)) || (rc = check_lists(a->sym))) return rc;
+  P2PDev d1 = d, d2 = d;
+  use_lists(d1, a->one);
+  use_lists(d2, a->sym);
+  if (a->precision == 0) {
+    d2.urev = static_cast<double*>(u);
+    if (a->one->nitems > 0 && (rc = launch_p2p_prec<double, false>(a->kernel, d1, a->one->nitems, st, mfma_min)))
+      return rc;
+    return launch_p2p_sym<double, false>(a->kernel, d2, a->sym->nitems, st, a->sym_unroll);
+  }
+  d2.urev = a->urev;
+  if ((rc = skf_fill_zero(a->urev, a->urev_len, 0, st))) return rc;
+  if (a->one->nitems > 0 && (rc = launch_p2p_prec<float, true>(a->kernel, d1, a->one->nitems, st, mfma_min)))
+    return rc;
+  if ((rc = launch_p2p_sym<float, true>(a->kernel, d2, a->sym->nitems, st, a->sym_unroll))) return rc;
+  const long long n = a->urev_len;
+  hipLaunchKernelGGL(fold_rev_kernel<float>, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, static_cast<float*>(u),
+                     a->urev, n);
+  SKF_LAUNCHED();
+  return SKF_OK;
 }
 
 // ---------------------------------------------------- interpolation kernels

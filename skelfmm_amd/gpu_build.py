@@ -8,12 +8,13 @@ Two paths for the boxes of one tree level, both in FP64 (complex128 for Helmholt
   reference's adaptive ID (``skelfmm/util.py:13-51``) with early stopping instead
   of a full factorization.
 * ``sketched`` (the learned-rank build): given a predicted rank ``k_b`` per box, a
-  Gaussian sketch ``Y_b = Omega A_b`` with ``l_b = ceil(1.25 k_b) + 8`` rows (one
+  Gaussian sketch ``Y_b = Omega A_b`` with ``l_b = ceil(1.5 k_b) + 16`` rows (one
   batched GEMM) replaces ``A_b`` in the pivoted QR, which then runs on ``l_b`` rows
-  instead of ``m``. A box is accepted when the QR of its sketch reaches the rank
-  cutoff within ``l_b`` steps and an independent 8-row sketch of the residual
-  ``A[:, red] - A[:, skel] T`` stays within ``10 tol`` of the sketched block norm
-  (the proxy-residual check); every other box falls back to the adaptive path.
+  instead of ``m``, with the cutoff tightened to ``tol / 4`` (a Gaussian sketch blurs
+  it). A box is accepted when the QR of its sketch reaches the cutoff within ``l_b``
+  steps and an independent 16-row sketch of the residual ``A[:, red] - A[:, skel] T``
+  stays within ``4 tol`` of the largest sketched column norm (the proxy-residual
+  check, ``skf_id_residual``); every other box falls back to the adaptive path.
 
 Results are returned in the CPU build's convention: ``(rank, idx, proj)`` with
 ``idx`` the column pivots and ``proj = T``.
@@ -46,15 +47,16 @@ def _unit_proxy(operator, ndim, nproxy):
     return out
 
 
-def accept_sketch(Z, result, lrows, tol, cap=None, slack=10.0):
+def accept_sketch(Z, result, lrows, tol, cap=None, accept=4.0):
     """Acceptance test of a sketched ID ``result = (rank, idx, T)`` from ``lrows`` sketch rows.
 
     Rejected when the sketch's QR ran out of rows before the rank cutoff
     (``rank >= lrows``: the predicted rank was too low -- unless ``lrows``
     already reaches ``cap = min(m, n)``, the largest possible rank), or when the residual
     ``Z[:, red] - Z[:, skel] T`` of an independent sketch ``Z`` of the proxy
-    matrix has a column norm above ``slack * tol`` times the largest column norm
-    of ``Z`` (the proxy-residual check at ``tol``).
+    matrix has a column norm above ``accept * tol`` times the largest column norm
+    of ``Z`` (the proxy-residual check at ``tol``). Host statement of the rule that
+    ``GPUSkeletonizer.sketched`` applies to the residuals ``skf_id_residual`` returns.
     """
     rk, idx, T = result
     if rk >= lrows and (cap is None or lrows < cap):
@@ -65,7 +67,7 @@ def accept_sketch(Z, result, lrows, tol, cap=None, slack=10.0):
     skel, red = Z[:, idx[:rk]], Z[:, idx[rk:]]
     E = red - skel @ T if rk > 0 else red
     den = float(np.max(np.linalg.norm(Z, axis=0)))
-    return float(np.max(np.linalg.norm(E, axis=0))) <= slack * tol * max(den, 1e-300)
+    return float(np.max(np.linalg.norm(E, axis=0))) <= accept * tol * max(den, 1e-300)
 
 
 class GPUSkeletonizer:
@@ -91,8 +93,20 @@ class GPUSkeletonizer:
         self.gen = torch.Generator(device=self.device)
         self.gen.manual_seed(seed)
         self.timings = {}
+        self.profile = False  # True: synchronize and time every phase (proxy, gemm, qr, check, host)
+        self._t = None
 
     # ----------------------------------------------------------------- common
+    def _phase(self, name):
+        """Close the running phase under ``name`` (only when profiling)."""
+        if not self.profile:
+            return
+        torch.cuda.synchronize(self.device)
+        now = perf_counter()
+        if self._t is not None:
+            self.timings[name] = self.timings.get(name, 0.0) + now - self._t
+        self._t = now
+
     def _stream(self):
         return hip.stream_handle(self.device)
 
@@ -142,54 +156,71 @@ class GPUSkeletonizer:
             1 if self.stacked else 0, 1 if self.cplx else 0, hip.ptr(A), hip.ptr(t_aoff), int(lda), len(cands),
             int(ncols.max()), self._stream()))
         keep = (t_idx, t_ioff, t_n, t_c, t_side, t_unit)
+        self._phase("proxy")
         return A, t_aoff, a_off, m, ncols, keep
 
-    def _qr_trsm(self, A, t_aoff, lda, rows, ncols, kmax):
-        """Batched pivoted QR (early stop at the cutoff) + triangular solves; host results."""
+    def _qr_trsm_dev(self, A, t_aoff, lda, rows, ncols, kmax, tol):
+        """Batched pivoted QR (early stop at ``tol``) + triangular solves, results left on the device.
+
+        ``lda``: per-box row strides (None: rows of ``ncols`` entries)."""
         dev = self.device
         nb = len(ncols)
         m_arr = np.asarray(rows, dtype=np.int32)
-        t_m = torch.as_tensor(m_arr, device=dev)
-        t_n = torch.as_tensor(np.asarray(ncols, dtype=np.int32), device=dev)
-        t_k = torch.as_tensor(np.asarray(kmax, dtype=np.int32), device=dev)
+        ncols = np.asarray(ncols, dtype=np.int64)
+        d = {"m": torch.as_tensor(m_arr, device=dev), "n": torch.as_tensor(ncols.astype(np.int32), device=dev),
+             "k": torch.as_tensor(np.asarray(kmax, dtype=np.int32), device=dev)}
+        d["lda"] = None if lda is None else torch.as_tensor(np.asarray(lda, dtype=np.int32), device=dev)
         p_off = np.concatenate(([0], np.cumsum(ncols)[:-1])).astype(np.int64)
-        t_poff = torch.as_tensor(p_off, device=dev)
-        perm = torch.empty(int(np.sum(ncols)), dtype=torch.int32, device=dev)
-        rdiag = torch.zeros(int(np.sum(ncols)), dtype=torch.float64, device=dev)
-        rank = torch.empty(nb, dtype=torch.int32, device=dev)
-        conv = torch.empty(nb, dtype=torch.int32, device=dev)
-        max_m, max_n = int(m_arr.max()), int(np.max(ncols))
+        d["p_off"] = torch.as_tensor(p_off, device=dev)
+        d["perm"] = torch.empty(max(1, int(ncols.sum())), dtype=torch.int32, device=dev)
+        d["rdiag"] = torch.zeros(max(1, int(ncols.sum())), dtype=torch.float64, device=dev)
+        d["rank"] = torch.empty(nb, dtype=torch.int32, device=dev)
+        d["conv"] = torch.empty(nb, dtype=torch.int32, device=dev)
+        max_m, max_n = int(m_arr.max()), int(ncols.max())
         hip.check_build(self.L.skf_pqr_batched(
-            hip.ptr(A), hip.ptr(t_aoff), int(lda), hip.ptr(t_m), hip.ptr(t_n), hip.ptr(t_k), self.tol, hip.ptr(rank),
-            hip.ptr(conv), hip.ptr(perm), hip.ptr(t_poff), hip.ptr(rdiag), nb, max_m, max_n, 1 if self.cplx else 0,
-            self._stream()))
-        r_h = rank.cpu().numpy().astype(np.int64)
-        c_h = conv.cpu().numpy().astype(bool)
-        nc = np.maximum(np.asarray(ncols, dtype=np.int64) - r_h, 0)
+            hip.ptr(A), hip.ptr(t_aoff), hip.ptr(d["lda"]), hip.ptr(d["m"]), hip.ptr(d["n"]), hip.ptr(d["k"]), float(tol),
+            hip.ptr(d["rank"]), hip.ptr(d["conv"]), hip.ptr(d["perm"]), hip.ptr(d["p_off"]), hip.ptr(d["rdiag"]), nb,
+            max_m, max_n, 1 if self.cplx else 0, self._stream()))
+        self._phase("qr")
+        r_h = d["rank"].cpu().numpy().astype(np.int64)
+        c_h = d["conv"].cpu().numpy().astype(bool)
+        nc = np.maximum(ncols - r_h, 0)
         sizes = np.where(c_h, r_h * nc, 0)
         t_off = np.concatenate(([0], np.cumsum(sizes)[:-1])).astype(np.int64)
-        T = torch.zeros(max(1, int(sizes.sum())) * (2 if self.cplx else 1), dtype=torch.float64, device=dev)
-        rank_solve = torch.as_tensor(np.where(c_h, r_h, 0).astype(np.int32), device=dev)
-        t_toff = torch.as_tensor(t_off, device=dev)
+        d["T"] = torch.zeros(max(1, int(sizes.sum())) * (2 if self.cplx else 1), dtype=torch.float64, device=dev)
+        d["rank_solve"] = torch.as_tensor(np.where(c_h, r_h, 0).astype(np.int32), device=dev)
+        d["t_off"] = torch.as_tensor(t_off, device=dev)
         hip.check_build(self.L.skf_trsm_batched(
-            hip.ptr(A), hip.ptr(t_aoff), int(lda), hip.ptr(t_n), hip.ptr(rank_solve), hip.ptr(T), hip.ptr(t_toff), nb,
-            max(1, int(nc.max()) if nc.size else 1), 1 if self.cplx else 0, self._stream()))
-        perm_h = perm.cpu().numpy().astype(np.int64)
-        T_h = T.cpu().numpy()
+            hip.ptr(A), hip.ptr(t_aoff), hip.ptr(d["lda"]), hip.ptr(d["n"]), hip.ptr(d["rank_solve"]), hip.ptr(d["T"]),
+            hip.ptr(d["t_off"]), nb, max(1, int(nc.max()) if nc.size else 1), 1 if self.cplx else 0, self._stream()))
+        host = {"rank": r_h, "conv": c_h, "p_off": p_off, "t_off": t_off, "ncols": ncols}
+        self._phase("trsm")
+        return d, host
+
+    def _results(self, d, host, keep):
+        """Host (rank, idx, proj) per box where ``keep`` holds, else None."""
+        perm_h = d["perm"].cpu().numpy().astype(np.int64)
+        T_h = d["T"].cpu().numpy()
         if self.cplx:
             T_h = T_h.view(np.complex128)
         out = []
-        for i in range(nb):
-            n = int(ncols[i])
-            r = int(r_h[i])
-            idx = perm_h[p_off[i]:p_off[i] + n]
-            if not c_h[i]:
+        for i in range(len(host["ncols"])):
+            if not keep[i]:
                 out.append(None)
                 continue
-            proj = T_h[t_off[i]:t_off[i] + r * (n - r)].reshape(r, n - r) if r * (n - r) > 0 else \
+            n, r = int(host["ncols"][i]), int(host["rank"][i])
+            po, to = host["p_off"][i], host["t_off"][i]
+            idx = perm_h[po:po + n]
+            proj = T_h[to:to + r * (n - r)].reshape(r, n - r) if r * (n - r) > 0 else \
                 np.zeros((r, n - r), dtype=np.complex128 if self.cplx else np.float64)
             out.append((r, idx, np.ascontiguousarray(proj)))
+        self._phase("host")
         return out
+
+    def _qr_trsm(self, A, t_aoff, lda, rows, ncols, kmax):
+        d, host = self._qr_trsm_dev(A, t_aoff, None if lda == 0 else np.full(len(ncols), lda), rows, ncols, kmax,
+                                    self.tol)
+        return self._results(d, host, host["conv"])
 
     def _lds_ok(self, rows, n):
         es = 16 if self.cplx else 8
@@ -199,6 +230,7 @@ class GPUSkeletonizer:
     def adaptive(self, cands, centers, length, nproxy):
         """Adaptive IDs of a level's boxes (None where the QR workspace does not fit LDS)."""
         t0 = perf_counter()
+        self._t = perf_counter()
         unitP = _unit_proxy(self.op, self.ndim, nproxy).shape[0]
         m = 2 * unitP if self.stacked else unitP
         results = [None] * len(cands)
@@ -215,17 +247,35 @@ class GPUSkeletonizer:
         return results
 
     # ---------------------------------------------------------------- sketched
-    def sketched(self, cands, centers, length, nproxy, k_pred, oversample=1.25, extra=8, check_rows=8):
-        """Learned-rank IDs from Gaussian sketches; None where the box must fall back."""
+    def sketched(self, cands, centers, length, nproxy, k_pred, oversample=1.25, extra=8, cut=0.25, accept=4.0,
+                 check_rows=16, direct=0.9):
+        """Learned-rank IDs from Gaussian sketches; None where the box must fall back.
+
+        Box b is sketched with ``l_b = min(m, n_b, ceil(oversample k_b) + extra)`` rows; the
+        pivoted QR of the sketch stops at ``|R_ii| <= cut * tol * |R_00|`` (the sketch blurs the
+        cutoff, a tighter one keeps the error at the adaptive ID's level). The ID is accepted
+        when the QR stopped before running out of rows and the residual on an independent
+        ``check_rows``-row sketch is within ``accept * tol`` (``accept_sketch``, computed on
+        the GPU by ``skf_id_residual``). Boxes predicted to be nearly incompressible
+        (``k_b >= direct * min(m, n_b)``: a sketch that size distorts too much to be
+        accepted) are not sketched; they return ``"direct"`` and take the adaptive path.
+        """
         t0 = perf_counter()
+        self._t = perf_counter()
         unitP = _unit_proxy(self.op, self.ndim, nproxy).shape[0]
         m = 2 * unitP if self.stacked else unitP
         nb = len(cands)
-        ncols_all = np.array([len(c) for c in cands])
-        lrows = np.minimum(np.minimum(ncols_all, m), np.ceil(oversample * np.asarray(k_pred)).astype(np.int64) + extra)
-        lrows = np.maximum(lrows, 1)
+        ncols_all = np.array([len(c) for c in cands], dtype=np.int64)
+        want = np.ceil(oversample * np.asarray(k_pred, dtype=np.float64)).astype(np.int64) + extra
+        lrows = np.maximum(np.minimum(np.minimum(ncols_all, m), want), 1)
         results = [None] * nb
-        ok = [i for i in range(nb) if self._lds_ok(m, int(ncols_all[i]))]
+        kp = np.asarray(k_pred, dtype=np.float64)
+        ok = []
+        for i in range(nb):
+            if kp[i] >= direct * min(m, int(ncols_all[i])):
+                results[i] = "direct"
+            elif self._lds_ok(m, int(ncols_all[i])):
+                ok.append(i)
         # group boxes of similar width so the padded batch wastes little
         order = sorted(ok, key=lambda i: ncols_all[i])
         groups, cur = [], []
@@ -239,24 +289,61 @@ class GPUSkeletonizer:
         if cur:
             groups.append(cur)
         cdt = torch.complex128 if self.cplx else torch.float64
+        # batches of groups whose sketches fit the budget: one QR / TRSM / check launch per batch
+        batches, cur, acc = [], [], 0
         for g in groups:
-            nmax = int(ncols_all[g].max())
-            lmax = int(lrows[g].max())
-            A, _, _, m_, ncols, keep = self._proxy([cands[i] for i in g], centers[g], length, nproxy, lda=nmax)
-            Ab = A.view(cdt) if self.cplx else A
-            Ab = Ab.view(len(g), m_, nmax)
-            Om = torch.randn(lmax + check_rows, m_, dtype=torch.float64, device=self.device, generator=self.gen)
-            Om = Om.to(cdt) if self.cplx else Om
-            YZ = torch.matmul(Om, Ab)  # (nb, lmax + check_rows, nmax)
-            Y = YZ[:, :lmax].contiguous()
-            Z = YZ[:, lmax:].cpu().numpy()
-            Yr = torch.view_as_real(Y).reshape(-1) if self.cplx else Y.reshape(-1)
-            y_off = torch.as_tensor(np.arange(len(g), dtype=np.int64) * lmax * nmax, device=self.device)
-            res = self._qr_trsm(Yr, y_off, nmax, lrows[g], ncols, lrows[g])
-            for j, (i, r) in enumerate(zip(g, res)):
-                if r is not None and accept_sketch(Z[j, :, :int(ncols_all[i])], r, int(lrows[i]), self.tol,
-                                                cap=min(m_, int(ncols_all[i]))):
-                    results[i] = r
-            del A, Ab, Y, YZ, keep
+            b = len(g) * (int(lrows[g].max()) + check_rows) * int(ncols_all[g].max()) * es
+            if cur and acc + b > self.budget:
+                batches.append(cur)
+                cur, acc = [], 0
+            cur.append(g)
+            acc += b
+        if cur:
+            batches.append(cur)
+        for batch in batches:
+            boxes = [i for g in batch for i in g]
+            ysz = [len(g) * int(lrows[g].max()) * int(ncols_all[g].max()) for g in batch]
+            zsz = [len(g) * check_rows * int(ncols_all[g].max()) for g in batch]
+            Ybuf = torch.empty(sum(ysz), dtype=cdt, device=self.device)
+            Zbuf = torch.empty(sum(zsz), dtype=cdt, device=self.device)
+            y_off, z_off, lda = [], [], []
+            yb = zb = 0
+            for g, ys, zs in zip(batch, ysz, zsz):
+                nmax = int(ncols_all[g].max())
+                lmax = int(lrows[g].max())
+                A, _, _, m_, _, keep = self._proxy([cands[i] for i in g], centers[g], length, nproxy, lda=nmax)
+                Ab = (A.view(cdt) if self.cplx else A).view(len(g), m_, nmax)
+                Om = torch.randn(lmax + check_rows, m_, dtype=torch.float64, device=self.device, generator=self.gen)
+                Om = Om.to(cdt) if self.cplx else Om
+                self._phase("host")
+                torch.matmul(Om[:lmax], Ab, out=Ybuf[yb:yb + ys].view(len(g), lmax, nmax))
+                torch.matmul(Om[lmax:], Ab, out=Zbuf[zb:zb + zs].view(len(g), check_rows, nmax))
+                del A, Ab, keep
+                self._phase("gemm")
+                y_off.extend(yb + j * lmax * nmax for j in range(len(g)))
+                z_off.extend(zb + j * check_rows * nmax for j in range(len(g)))
+                lda.extend([nmax] * len(g))
+                yb += ys
+                zb += zs
+            Yr = torch.view_as_real(Ybuf).reshape(-1) if self.cplx else Ybuf
+            Zr = torch.view_as_real(Zbuf).reshape(-1) if self.cplx else Zbuf
+            t_yoff = torch.as_tensor(np.asarray(y_off, dtype=np.int64), device=self.device)
+            t_zoff = torch.as_tensor(np.asarray(z_off, dtype=np.int64), device=self.device)
+            lg = lrows[boxes]
+            ncb = ncols_all[boxes]
+            d, host = self._qr_trsm_dev(Yr, t_yoff, lda, lg, ncb, lg, cut * self.tol)
+            res = torch.empty(len(boxes), dtype=torch.float64, device=self.device)
+            zn = torch.empty(len(boxes), dtype=torch.float64, device=self.device)
+            hip.check_build(self.L.skf_id_residual(
+                hip.ptr(Zr), hip.ptr(t_zoff), hip.ptr(d["lda"]), check_rows, hip.ptr(d["n"]), hip.ptr(d["rank_solve"]),
+                hip.ptr(d["perm"]), hip.ptr(d["p_off"]), hip.ptr(d["T"]), hip.ptr(d["t_off"]), len(boxes),
+                1 if self.cplx else 0, hip.ptr(res), hip.ptr(zn), self._stream()))
+            res_h, zn_h = res.cpu().numpy(), zn.cpu().numpy()
+            self._phase("check")
+            cap = np.minimum(m, ncb)
+            good = host["conv"] & ((host["rank"] < lg) | (lg >= cap)) & (res_h <= accept * self.tol * np.maximum(zn_h, 1e-300))
+            for i, r in zip(boxes, self._results(d, host, good)):
+                results[i] = r
+            del Ybuf, Zbuf, d
         self.timings["sketched"] = self.timings.get("sketched", 0.0) + perf_counter() - t0
         return results

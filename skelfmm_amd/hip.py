@@ -13,7 +13,7 @@ import os
 
 import torch
 
-__all__ = ["lib", "check", "stream_handle", "P2PArgs", "LevelArgs", "PlanDesc", "p2p_blocks", "LIB_PATH",
+__all__ = ["lib", "check", "stream_handle", "P2PArgs", "P2PLists", "LevelArgs", "PlanDesc", "p2p_blocks", "LIB_PATH",
            "EXPORTED_SYMBOLS"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libskelfmm_hip.so")
@@ -35,6 +35,7 @@ EXPORTED_SYMBOLS = (
     "skf_proxy_matrices",
     "skf_pqr_batched",
     "skf_trsm_batched",
+    "skf_id_residual",
     "skf_mlp_param_count",
     "skf_mlp_forward",
     "skf_mlp_backward",
@@ -45,6 +46,14 @@ _vp = ctypes.c_void_p
 _i = ctypes.c_int
 
 
+class P2PLists(ctypes.Structure):
+    _fields_ = [
+        ("ngroups", _i), ("g_pt", _vp), ("g_n", _vp), ("g_out", _vp), ("g_box", _vp),
+        ("nentries", _i), ("e_pt", _vp), ("e_n", _vp), ("e_q", _vp), ("e_box", _vp), ("e_flags", _vp),
+        ("nitems", _i), ("items", _vp),
+    ]
+
+
 class P2PArgs(ctypes.Structure):
     _fields_ = [
         ("kernel", _i), ("precision", _i), ("width", _i), ("kappa", ctypes.c_double),
@@ -53,6 +62,8 @@ class P2PArgs(ctypes.Structure):
         ("ngroups", _i), ("g_pt", _vp), ("g_n", _vp), ("g_out", _vp), ("g_box", _vp),
         ("nentries", _i), ("e_pt", _vp), ("e_n", _vp), ("e_q", _vp), ("e_box", _vp), ("e_flags", _vp),
         ("nitems", _i), ("items", _vp), ("mfma_min_channels", _i),
+        ("one", ctypes.POINTER(P2PLists)), ("sym", ctypes.POINTER(P2PLists)), ("urev", _vp),
+        ("urev_len", ctypes.c_longlong), ("sym_unroll", _i),
     ]
 
 
@@ -100,9 +111,10 @@ def lib():
     L.skf_build_last_error.restype = ctypes.c_char_p
     L.skf_proxy_matrices.argtypes = [_i, _i, ctypes.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i,
                                      _vp, _vp, _i, _i, _i, _vp]
-    L.skf_pqr_batched.argtypes = [_vp, _vp, _i, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _vp, _i, _i, _i,
-                                  _i, _vp]
-    L.skf_trsm_batched.argtypes = [_vp, _vp, _i, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]
+    L.skf_pqr_batched.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _vp, _vp, _vp, _vp, _i, _i,
+                                  _i, _i, _vp]
+    L.skf_trsm_batched.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _vp]
+    L.skf_id_residual.argtypes = [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp, _vp, _vp]
     L.skf_mlp_forward.argtypes = [_vp, _vp, _i, _vp, _vp]
     L.skf_mlp_backward.argtypes = [_vp, _vp, _vp, _i, _vp, _vp, _vp]
     L.skf_adam_step.argtypes = [_vp, _vp, _vp, _vp, _i, ctypes.c_float, ctypes.c_float, ctypes.c_float,

@@ -171,6 +171,52 @@ def build_layout(fmm, max_sources=16384):
     return L
 
 
+def _empty_lists(tpi):
+    NL = NearLists()
+    for k in NearLists.FIELDS:
+        setattr(NL, k, np.zeros(0, dtype=np.int64 if k in ("g_e0", "g_e1") else np.int32))
+    NL.items = np.zeros((0, 4), dtype=np.int32)
+    NL.item_cost = np.zeros(0, dtype=np.int64)
+    NL.interactions = 0
+    NL.tpi = tpi
+    return NL
+
+
+def _merge_lists(parts, slot_base, box_base, nslots):
+    """Concatenate per-instance near-field lists (objects with the NearLists fields)."""
+    M = NearLists()
+    g_base = np.cumsum([0] + [P.g_pt.shape[0] for P in parts])[:-1]
+    e_base = np.cumsum([0] + [P.e_pt.shape[0] for P in parts])[:-1]
+    M.g_pt = np.concatenate([P.g_pt + sb for P, sb in zip(parts, slot_base)]).astype(np.int32)
+    M.g_n = np.concatenate([P.g_n for P in parts]).astype(np.int32)
+    M.g_kind = np.concatenate([P.g_kind for P in parts]).astype(np.int32)
+    M.g_out = (M.g_pt + M.g_kind * nslots).astype(np.int32)
+    M.g_box = np.concatenate([P.g_box + bb for P, bb in zip(parts, box_base)]).astype(np.int32)
+    M.e_pt = np.concatenate([P.e_pt + sb for P, sb in zip(parts, slot_base)]).astype(np.int32)
+    M.e_n = np.concatenate([P.e_n for P in parts]).astype(np.int32)
+    M.e_qkind = np.concatenate([P.e_qkind for P in parts]).astype(np.int32)
+    M.e_q = (M.e_pt + M.e_qkind * nslots).astype(np.int32)
+    M.e_box = np.concatenate([P.e_box + bb for P, bb in zip(parts, box_base)]).astype(np.int32)
+    M.e_flags = np.concatenate([P.e_flags for P in parts]).astype(np.int32)
+    M.g_e0 = np.concatenate([P.g_e0 + eb for P, eb in zip(parts, e_base)]).astype(np.int64)
+    M.g_e1 = np.concatenate([P.g_e1 + eb for P, eb in zip(parts, e_base)]).astype(np.int64)
+    items, costs = [], []
+    for P, gb, eb in zip(parts, g_base, e_base):
+        it = P.items.copy().reshape(-1, 4)
+        it[:, 0] += gb
+        it[:, 2] += eb
+        it[:, 3] += eb
+        items.append(it)
+        costs.append(P.item_cost)
+    items = np.concatenate(items) if items else np.zeros((0, 4), np.int32)
+    costs = np.concatenate(costs) if costs else np.zeros(0, np.int64)
+    order = np.argsort(-costs, kind="stable")
+    M.items = np.ascontiguousarray(items[order]).astype(np.int32)
+    M.item_cost = costs[order]
+    M.interactions = int(sum(P.interactions for P in parts))
+    return M
+
+
 def merge_layouts(layouts):
     """One layout for several independent instances (same operator and dimension).
 
@@ -236,45 +282,27 @@ def merge_layouts(layouts):
     M.up = merge_stages([L.up for L in layouts], True)
     M.down = merge_stages([L.down for L in layouts], False)
 
-    g_base = np.cumsum([0] + [L.g_pt.shape[0] for L in layouts])[:-1]
-    e_base = np.cumsum([0] + [L.e_pt.shape[0] for L in layouts])[:-1]
-    M.g_pt = np.concatenate([L.g_pt + sb for L, sb in zip(layouts, slot_base)]).astype(np.int32)
-    M.g_n = np.concatenate([L.g_n for L in layouts]).astype(np.int32)
-    M.g_kind = np.concatenate([L.g_kind for L in layouts]).astype(np.int32)
-    M.g_out = (M.g_pt + M.g_kind * M.nslots).astype(np.int32)
-    M.g_box = np.concatenate([L.g_box + bb for L, bb in zip(layouts, box_base)]).astype(np.int32)
-    M.e_pt = np.concatenate([L.e_pt + sb for L, sb in zip(layouts, slot_base)]).astype(np.int32)
-    M.e_n = np.concatenate([L.e_n for L in layouts]).astype(np.int32)
-    M.e_qkind = np.concatenate([L.e_qkind for L in layouts]).astype(np.int32)
-    M.e_q = (M.e_pt + M.e_qkind * M.nslots).astype(np.int32)
-    M.e_box = np.concatenate([L.e_box + bb for L, bb in zip(layouts, box_base)]).astype(np.int32)
-    M.e_flags = np.concatenate([L.e_flags for L in layouts]).astype(np.int32)
-    M.g_e0 = np.concatenate([L.g_e0 + eb for L, eb in zip(layouts, e_base)])
-    M.g_e1 = np.concatenate([L.g_e1 + eb for L, eb in zip(layouts, e_base)])
-    items = []
-    costs = []
-    for L, gb, eb in zip(layouts, g_base, e_base):
-        it = L.items.copy()
-        it[:, 0] += gb
-        it[:, 2] += eb
-        it[:, 3] += eb
-        items.append(it)
-        costs.append(L.item_cost)
-    items = np.concatenate(items)
-    costs = np.concatenate(costs)
-    order = np.argsort(-costs, kind="stable")
-    M.items = np.ascontiguousarray(items[order]).astype(np.int32)
-    M.item_cost = costs[order]
+    full = _merge_lists(layouts, slot_base, box_base, M.nslots)
+    for k in NearLists.FIELDS + ("items", "item_cost"):
+        setattr(M, k, getattr(full, k))
+    M.split = None
+    if any(L.split is not None for L in layouts):
+        ones = [L.split[0] if L.split is not None else L for L in layouts]
+        syms = [L.split[1] if L.split is not None else _empty_lists(SYM_TARGETS_PER_ITEM) for L in layouts]
+        M.split = (_merge_lists(ones, slot_base, box_base, M.nslots),
+                   _merge_lists(syms, slot_base, box_base, M.nslots))
+        M.split[0].tpi, M.split[1].tpi = TARGETS_PER_ITEM, SYM_TARGETS_PER_ITEM
+    M.evaluations = int(sum(L.evaluations for L in layouts))
     M.interactions = int(sum(L.interactions for L in layouts))
     M.max_sources = int(layouts[0].max_sources)
     return M
 
 
-def make_items(g_n, g_e0, g_e1, e_n, max_sources):
+def make_items(g_n, g_e0, g_e1, e_n, max_sources, targets_per_item=TARGETS_PER_ITEM):
     """Near-field work items {group, first target, first entry, end entry}, heaviest first.
 
-    Each item covers up to 64 targets of one group and a run of that group's source
-    entries holding at most ``max_sources`` points (at least one entry).
+    Each item covers up to ``targets_per_item`` targets of one group and a run of that
+    group's source entries holding at most ``max_sources`` points (at least one entry).
     """
     items = []
     costs = []
@@ -290,8 +318,8 @@ def make_items(g_n, g_e0, g_e1, e_n, max_sources):
             ranges.append((a, b))
             a = b
         ng = int(g_n[g])
-        for t0 in range(0, ng, TARGETS_PER_ITEM):
-            nt = min(TARGETS_PER_ITEM, ng - t0)
+        for t0 in range(0, ng, targets_per_item):
+            nt = min(targets_per_item, ng - t0)
             for a, b in ranges:
                 items.append((g, t0, a, b))
                 costs.append(nt * int(cums[b] - cums[a]))
@@ -304,8 +332,78 @@ def make_items(g_n, g_e0, g_e1, e_n, max_sources):
 def resplit(L, max_sources):
     """Rebuild a layout's near-field work items for another source-run length (in place)."""
     L.items, L.item_cost = make_items(L.g_n, L.g_e0, L.g_e1, L.e_n, max_sources)
+    if L.split is not None:
+        for part in L.split:
+            part.items, part.item_cost = make_items(part.g_n, part.g_e0, part.g_e1, part.e_n, max_sources, part.tpi)
     L.max_sources = int(max_sources)
     return L
+
+
+class NearLists:
+    """Near-field groups (target sets), entries (source sets) and work items."""
+
+    FIELDS = ("g_pt", "g_n", "g_out", "g_box", "g_kind", "g_e0", "g_e1", "e_pt", "e_n", "e_q", "e_qkind", "e_box",
+              "e_flags")
+
+
+# kernels with K(x, y) = K(y, x) (single layers): pairs of lists that run in both directions are
+# evaluated once by the symmetric near-field kernel
+SYMMETRIC_KERNEL_IDS = (0, 1, 2, 3)
+SYM_TARGETS_PER_ITEM = 512
+
+
+def _lists_from_rows(R, off, bs, rk, nslots, max_sources, tpi):
+    """Groups / entries / items from rows (tbox, tkind, sbox, skind, qkind, neg) with nonzero sets."""
+    NL = NearLists()
+    NL.tpi = int(tpi)
+    order = np.lexsort((R[:, 2], R[:, 1], R[:, 0]))
+    R = R[order]
+    scount = np.where(R[:, 3] == 0, bs[R[:, 2]], rk[R[:, 2]])
+    self_pair = (R[:, 0] == R[:, 2]) & (R[:, 1] == R[:, 3])
+    NL.e_pt = off[R[:, 2]].astype(np.int32)
+    NL.e_n = scount.astype(np.int32)
+    NL.e_qkind = R[:, 4].astype(np.int32)
+    NL.e_q = (off[R[:, 2]] + R[:, 4] * nslots).astype(np.int32)
+    NL.e_flags = (R[:, 5] * 1 + self_pair * 2).astype(np.int32)
+    NL.e_box = R[:, 2].astype(np.int32)
+    key = R[:, 0] * 2 + R[:, 1]
+    gkeys, gstart, gcount = np.unique(key, return_index=True, return_counts=True)
+    g_box = gkeys // 2
+    g_kind = gkeys % 2
+    NL.g_pt = off[g_box].astype(np.int32)
+    NL.g_n = np.where(g_kind == 0, bs[g_box], rk[g_box]).astype(np.int32)
+    NL.g_out = (off[g_box] + g_kind * nslots).astype(np.int32)
+    NL.g_box = g_box.astype(np.int32)
+    NL.g_kind = g_kind.astype(np.int32)
+    NL.g_e0 = gstart.astype(np.int64)
+    NL.g_e1 = (gstart + gcount).astype(np.int64)
+    NL.items, NL.item_cost = make_items(NL.g_n, NL.g_e0, NL.g_e1, NL.e_n, max_sources, tpi)
+    tn = NL.g_n[np.searchsorted(gkeys, key)].astype(np.int64) if key.shape[0] else np.zeros(0, np.int64)
+    NL.interactions = int(np.sum(tn * NL.e_n.astype(np.int64)))
+    return NL
+
+
+def pair_rows(R, nboxes):
+    """Split rows into (one-sided rows, pair representatives).
+
+    Row (U, a, V, b, q, neg) pairs with (V, b, U, a, q', neg) for U != V when both are unique
+    and each charge kind equals its source set's kind (a set's charges and potentials then share
+    slots). The representative is the row whose target set orders first.
+    """
+    if R.shape[0] == 0:
+        return R, R
+    a = R[:, 0] * 2 + R[:, 1]
+    b = R[:, 2] * 2 + R[:, 3]
+    nb2 = 2 * int(nboxes)
+    key = (a * nb2 + b) * 2 + R[:, 5]
+    pkey = (b * nb2 + a) * 2 + R[:, 5]
+    uk, cnt = np.unique(key, return_counts=True)
+    single = np.isin(key, uk[cnt == 1])
+    ok = single & (R[:, 0] != R[:, 2]) & (R[:, 4] == R[:, 3])
+    okkeys = key[ok]
+    paired = ok & np.isin(pkey, okkeys)
+    rep = paired & (a < b)
+    return R[~paired], R[rep]
 
 
 def _near_field(L, fmm, off, bs, rk, max_sources):
@@ -338,42 +436,18 @@ def _near_field(L, fmm, off, bs, rk, max_sources):
 
     tcount = np.where(R[:, 1] == 0, bs[R[:, 0]], rk[R[:, 0]])
     scount = np.where(R[:, 3] == 0, bs[R[:, 2]], rk[R[:, 2]])
-    keep = (tcount > 0) & (scount > 0)
-    R = R[keep]
-    scount = scount[keep]
-    order = np.lexsort((R[:, 2], R[:, 1], R[:, 0]))
-    R = R[order]
-    scount = scount[order]
+    R = R[(tcount > 0) & (scount > 0)]
 
-    e_pt = off[R[:, 2]]
-    e_n = scount
-    e_q = off[R[:, 2]] + R[:, 4] * nslots
-    self_pair = (R[:, 0] == R[:, 2]) & (R[:, 1] == R[:, 3])
-    e_flags = R[:, 5] * 1 + self_pair * 2
-    e_box = R[:, 2]
-
-    key = R[:, 0] * 2 + R[:, 1]
-    gkeys, gstart, gcount = np.unique(key, return_index=True, return_counts=True)
-    g_box = gkeys // 2
-    g_kind = gkeys % 2
-    g_pt = off[g_box]
-    g_n = np.where(g_kind == 0, bs[g_box], rk[g_box])
-    g_out = off[g_box] + g_kind * nslots
-
-    L.g_e0 = gstart.astype(np.int64)
-    L.g_e1 = (gstart + gcount).astype(np.int64)
-    L.items, L.item_cost = make_items(g_n, L.g_e0, L.g_e1, e_n, max_sources)
-    L.g_pt = g_pt.astype(np.int32)
-    L.g_n = g_n.astype(np.int32)
-    L.g_out = g_out.astype(np.int32)
-    L.g_box = g_box.astype(np.int32)
-    L.g_kind = g_kind.astype(np.int32)
-    L.e_qkind = R[:, 4].astype(np.int32)
-    L.e_pt = e_pt.astype(np.int32)
-    L.e_n = e_n.astype(np.int32)
-    L.e_q = e_q.astype(np.int32)
-    L.e_box = e_box.astype(np.int32)
-    L.e_flags = e_flags.astype(np.int32)
-    # exact interaction count of one apply (targets x sources over every entry)
-    tn = g_n[np.searchsorted(gkeys, key)] if key.shape[0] else np.zeros(0, np.int64)
-    L.interactions = int(np.sum(tn.astype(np.int64) * e_n.astype(np.int64)))
+    full = _lists_from_rows(R, off, bs, rk, nslots, max_sources, TARGETS_PER_ITEM)
+    for k in NearLists.FIELDS + ("items", "item_cost", "interactions"):
+        setattr(L, k, getattr(full, k))
+    L.evaluations = L.interactions
+    L.split = None
+    if fmm.operator.kernel_id in SYMMETRIC_KERNEL_IDS:
+        one_rows, sym_rows = pair_rows(R, fmm.nboxes)
+        if sym_rows.shape[0]:
+            one = _lists_from_rows(one_rows, off, bs, rk, nslots, max_sources, TARGETS_PER_ITEM)
+            sym = _lists_from_rows(sym_rows, off, bs, rk, nslots, max_sources, SYM_TARGETS_PER_ITEM)
+            assert one.interactions + 2 * sym.interactions == L.interactions
+            L.split = (one, sym)
+            L.evaluations = one.interactions + sym.interactions

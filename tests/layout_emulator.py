@@ -2,7 +2,8 @@
 
 Executes exactly the index maps and work lists the HIP kernels consume
 (charge gather, upward gather + GEMV per stage, near-field work items with the
-SRC_NEG / SRC_SELF flags, downward parent step + GEMV^T, result scatter), with
+SRC_NEG / SRC_SELF flags -- or the symmetric split: one-sided rest plus pair entries
+evaluated in both directions -- downward parent step + GEMV^T, result scatter), with
 dense blocks from the oracle's clamped torch kernels. Agreement with the
 oracle's padded apply validates the layout on the CPU; agreement of the GPU
 with the oracle then isolates the kernels.
@@ -14,7 +15,32 @@ import torch
 from oracle.kernels import GEOM_SIDE, kernel_matrix_torch
 
 
-def emulate_apply(L, op_name, kappa, x):
+def _near(lists, tpi, P, G, side, op_name, kappa, q, u, dt, symmetric):
+    for g, t0, sb, se in lists.items:
+        nt = min(tpi, int(lists.g_n[g]) - t0)
+        tl = t0 + np.arange(nt)
+        ts = lists.g_pt[g] + tl
+        acc = np.zeros(nt, dtype=dt)
+        for s in range(sb, se):
+            ss = lists.e_pt[s] + np.arange(lists.e_n[s])
+            XT = torch.as_tensor(P[:, ts].T)[None]
+            XS = torch.as_tensor(P[:, ss].T)[None]
+            gt = torch.as_tensor(G[:, ts].T)[None] if side == "target" else None
+            gs = torch.as_tensor(G[:, ss].T)[None] if side == "source" else None
+            K = kernel_matrix_torch(op_name, XT, XS, kappa, False, geom_trg=gt, geom_src=gs)[0].numpy()
+            if lists.e_flags[s] & 2:
+                assert not symmetric, "self segments stay one-sided"
+                lj = np.arange(lists.e_n[s])
+                K = np.where(tl[:, None] == lj[None, :], 0.0, K)
+            sgn = -1.0 if (lists.e_flags[s] & 1) else 1.0
+            jq = lists.e_q[s] + np.arange(lists.e_n[s])
+            acc += sgn * (K @ q[jq])
+            if symmetric:  # reverse direction: potentials of the entry at its charge slots
+                u[jq] += sgn * (K.T @ q[lists.g_out[g] + tl])
+        u[lists.g_out[g] + tl] += acc
+
+
+def emulate_apply(L, op_name, kappa, x, split=False):
     x = np.asarray(x)
     cplx = np.iscomplexobj(x) or op_name in ("helmholtz_2d", "helmholtz_3d", "helmholtz_double_layer_3d",
                                              "helmholtz_adjoint_double_layer_3d")
@@ -35,24 +61,12 @@ def emulate_apply(L, op_name, kappa, x):
     P = L.points
     G = L.normals
     side = GEOM_SIDE.get(op_name)
-    for g, t0, sb, se in L.items:
-        nt = min(64, int(L.g_n[g]) - t0)
-        ts = L.g_pt[g] + t0 + np.arange(nt)
-        acc = np.zeros(nt, dtype=dt)
-        for s in range(sb, se):
-            ss = L.e_pt[s] + np.arange(L.e_n[s])
-            XT = torch.as_tensor(P[:, ts].T)[None]
-            XS = torch.as_tensor(P[:, ss].T)[None]
-            gt = torch.as_tensor(G[:, ts].T)[None] if side == "target" else None
-            gs = torch.as_tensor(G[:, ss].T)[None] if side == "source" else None
-            K = kernel_matrix_torch(op_name, XT, XS, kappa, False, geom_trg=gt, geom_src=gs)[0].numpy()
-            if L.e_flags[s] & 2:
-                li = t0 + np.arange(nt)
-                lj = np.arange(L.e_n[s])
-                K = np.where(li[:, None] == lj[None, :], 0.0, K)
-            v = K @ q[L.e_q[s] + np.arange(L.e_n[s])]
-            acc += -v if (L.e_flags[s] & 1) else v
-        u[L.g_out[g] + t0 + np.arange(nt)] += acc
+    if split:
+        assert L.split is not None
+        _near(L.split[0], 64, P, G, side, op_name, kappa, q, u, dt, False)
+        _near(L.split[1], L.split[1].tpi, P, G, side, op_name, kappa, q, u, dt, True)
+    else:
+        _near(L, 64, P, G, side, op_name, kappa, q, u, dt, False)
 
     for S in L.down:
         us = u[ns + S.gather_dst] + u[S.gather_src]

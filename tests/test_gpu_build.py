@@ -125,7 +125,7 @@ def test_batched_pivoted_qr_matches_scipy(cplx):
     rdiag = torch.zeros(sum(ns), dtype=torch.float64, device=dev)
     tol = 1e-7
     st = hip.stream_handle(dev)
-    hip.check_build(L.skf_pqr_batched(hip.ptr(buf), hip.ptr(d_aoff), 0, hip.ptr(d_m), hip.ptr(d_n), hip.ptr(d_k), tol,
+    hip.check_build(L.skf_pqr_batched(hip.ptr(buf), hip.ptr(d_aoff), None, hip.ptr(d_m), hip.ptr(d_n), hip.ptr(d_k), tol,
                                       hip.ptr(rank), hip.ptr(conv), hip.ptr(perm), hip.ptr(d_poff), hip.ptr(rdiag), nb,
                                       max(ms), max(ns), int(cplx), st))
     r_h = rank.cpu().numpy()
@@ -134,7 +134,7 @@ def test_batched_pivoted_qr_matches_scipy(cplx):
     t_off = np.concatenate(([0], np.cumsum(t_sizes)[:-1])).astype(np.int64)
     d_toff = t(t_off, torch.int64)
     T = torch.zeros(max(1, sum(t_sizes)) * es, dtype=torch.float64, device=dev)
-    hip.check_build(L.skf_trsm_batched(hip.ptr(buf), hip.ptr(d_aoff), 0, hip.ptr(d_n), hip.ptr(rank), hip.ptr(T),
+    hip.check_build(L.skf_trsm_batched(hip.ptr(buf), hip.ptr(d_aoff), None, hip.ptr(d_n), hip.ptr(rank), hip.ptr(T),
                                        hip.ptr(d_toff), nb, max(ns), int(cplx), st))
     torch.cuda.synchronize()
     Th = T.cpu().numpy()
@@ -151,6 +151,54 @@ def test_batched_pivoted_qr_matches_scipy(cplx):
         if r < n:
             E = A[:, piv[r:]] - A[:, piv[:r]] @ Tb
             assert np.linalg.norm(E, 2) <= 1e-8 * np.linalg.norm(A, 2)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_id_residual_kernel_matches_host(cplx):
+    """skf_id_residual against the host acceptance rule's residual on random sketches."""
+    from skelfmm_amd.gpu_build import accept_sketch
+
+    dev = _gpu()
+    L = hip.lib()
+    rng = np.random.default_rng(7)
+    c, lda = 16, 90
+    ns = np.array([90, 40, 7, 1, 60], dtype=np.int64)
+    rk = np.array([30, 40, 0, 1, 12], dtype=np.int64)
+    nb = len(ns)
+    g = lambda *sh: rng.normal(size=sh) + (1j * rng.normal(size=sh) if cplx else 0)
+    Z = g(nb, c, lda)
+    perms = [rng.permutation(int(n)) for n in ns]
+    Ts = [g(int(r), int(n - r)) for r, n in zip(rk, ns)]
+    p_off = np.concatenate(([0], np.cumsum(ns)[:-1])).astype(np.int64)
+    sizes = rk * (ns - rk)
+    t_off = np.concatenate(([0], np.cumsum(sizes)[:-1])).astype(np.int64)
+    Tflat = np.concatenate([T.ravel() for T in Ts] + [np.zeros(1, dtype=Z.dtype)])
+    as_dev = lambda a: torch.as_tensor(np.ascontiguousarray(a).view(np.float64) if cplx else a, device=dev)
+    d_Z, d_T = as_dev(Z.ravel()), as_dev(Tflat)
+    i32 = lambda a: torch.as_tensor(np.asarray(a, dtype=np.int32), device=dev)
+    i64 = lambda a: torch.as_tensor(np.asarray(a, dtype=np.int64), device=dev)
+    d_zoff, d_n, d_r, d_perm, d_poff, d_toff = (i64(np.arange(nb) * c * lda), i32(ns), i32(rk),
+                                                 i32(np.concatenate(perms)), i64(p_off), i64(t_off))
+    res = torch.empty(nb, dtype=torch.float64, device=dev)
+    zn = torch.empty(nb, dtype=torch.float64, device=dev)
+    d_lda = i32(np.full(nb, lda))
+    hip.check_build(L.skf_id_residual(hip.ptr(d_Z), hip.ptr(d_zoff), hip.ptr(d_lda), c, hip.ptr(d_n), hip.ptr(d_r),
+                                      hip.ptr(d_perm), hip.ptr(d_poff), hip.ptr(d_T), hip.ptr(d_toff), nb, int(cplx),
+                                      hip.ptr(res), hip.ptr(zn), hip.stream_handle(dev)))
+    torch.cuda.synchronize()
+    res_h, zn_h = res.cpu().numpy(), zn.cpu().numpy()
+    for b in range(nb):
+        n, r = int(ns[b]), int(rk[b])
+        Zb = Z[b, :, :n]
+        want_zn = np.max(np.linalg.norm(Zb, axis=0))
+        E = Zb[:, perms[b][r:]] - (Zb[:, perms[b][:r]] @ Ts[b] if r else 0)
+        want_res = np.max(np.linalg.norm(E, axis=0)) if n > r else 0.0
+        assert zn_h[b] == pytest.approx(want_zn, rel=1e-13)
+        assert res_h[b] == pytest.approx(want_res, rel=1e-12, abs=1e-300)
+        # the GPU numbers decide exactly as the host rule does
+        thr = res_h[b] / (1e-6 * want_zn) * 1.0001
+        assert accept_sketch(Zb, (r, perms[b], Ts[b]), n, 1e-6, cap=n, accept=thr)
+        assert n == r or not accept_sketch(Zb, (r, perms[b], Ts[b]), n, 1e-6, cap=n, accept=thr / 1.001)
 
 
 # --------------------------------------------------------------- builds
@@ -211,7 +259,7 @@ def test_forced_low_rank_falls_back_to_adaptive():
     assert f_a.bs_list[f_a.rank_list > 0].min() > 10
     f_l = _build(x, ops.LAPLACE_3D, 0.0, 1e-6, 128, "gpu", rank_model=_Low())
     st = f_l.gpu_stats
-    assert st["sketched"] == 0 and st["fallback"] > 0 and st["fallback"] == st["adaptive"]
+    assert st["sketched"] == 0 and st["direct"] == 0 and st["fallback"] > 0 and st["fallback"] == st["adaptive"]
     np.testing.assert_array_equal(f_l.rank_list, f_a.rank_list)
     for b in np.nonzero(f_a.rank_list)[0]:
         np.testing.assert_array_equal(f_l.cand[b], f_a.cand[b])
@@ -328,11 +376,15 @@ def test_learned_build_accuracy_and_fallback_rate():
     _gpu()
     model = lr.load_default()
     assert model is not None
-    x = np.random.default_rng(3).random((30000, 3))
-    f = _build(x, ops.HELMHOLTZ_3D, 10.0, 1e-6, 128, "gpu", rank_model=model)
+    # clustered cloud (16-component Gaussian mixture, sigma 0.02): many compressible boxes
+    rng = np.random.default_rng(3)
+    c = rng.random((16, 3))
+    x = np.clip(c[rng.choice(16, 100000, p=rng.dirichlet(np.ones(16)))] + 0.02 * rng.normal(size=(100000, 3)), 0, 1)
+    f = _build(x, ops.HELMHOLTZ_3D, 20.0, 1e-6, 256, "gpu", rank_model=model)
     st = f.gpu_stats
     rate = st["fallback"] / max(1, st["sketched"] + st["fallback"])
     e = _apply_err(f, x)
     print(f"learned build: {st}, fallback rate {rate:.2%}, apply error {e:.2e}")
-    assert rate < 0.2
+    assert st["sketched"] > 0 and rate < 0.15
+    assert st["adaptive"] == st["fallback"] + st["direct"]
     assert e < 1e-5

@@ -355,5 +355,42 @@ def test_autotune_picks_setting_and_keeps_parity(tmp_path, monkeypatch):
     y = dev.matvec(torch.as_tensor(q, device=_gpu())).cpu().numpy()
     ref = BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(q).numpy()
     assert rel_l2(y, ref) <= FP64_TOL
+    assert best["sym_unroll"] in (1, 2, 4) and dev.sym_unroll == best["sym_unroll"]
     again = autotune.tune(dev, 1, reps=1, splits=(2048, 16384))
     assert again["cached"] and again["max_sources"] == best["max_sources"]
+
+
+@pytest.mark.parametrize("name", ["laplace_2d_annulus", "laplace_3d_cube", "helmholtz_3d_cube", "helmholtz_2d_square"])
+@pytest.mark.parametrize("single", [False, True])
+def test_symmetric_near_field_matches_one_sided(name, single):
+    """Pairs evaluated once for both directions (p2p_sym_kernel) against the one-sided kernel:
+    FP64 to 1e-13 (only the summation order differs), FP32 form to 3e-6 of the result."""
+    from skelfmm_amd.batched import SkelFMMBatched
+
+    X, op, kappa, geom, leaf = _case(name)
+    print(f"case {name} single={single} N={X.shape[0]}", flush=True)
+    fmm = make_build(X, op, kappa=kappa, tol=1e-6, leaf_size=leaf, geometry=geom)
+    dev = SkelFMMBatched(fmm.tree, op, kappa=kappa).build_from_copy(fmm, use_single_precision=single)
+    assert dev.use_symmetric and dev.evaluations < dev.interactions
+    rs = np.random.RandomState(2)
+    q = rs.randn(X.shape[0])
+    if op.is_complex:
+        q = q + 1j * rs.randn(X.shape[0])
+    x = torch.as_tensor(q).to(_gpu())
+    y_sym = dev.matvec(x).cpu().numpy()
+    dev.set_symmetric(False)
+    y_one = dev.matvec(x).cpu().numpy()
+    dev.set_symmetric(True)
+    y_again = dev.matvec(x).cpu().numpy()
+    err = rel_l2(y_sym, y_one)
+    print(f"symmetric vs one-sided: {err:.2e}")
+    assert err <= (3e-6 if single else 1e-13)
+    assert rel_l2(y_again, y_sym) <= (1e-6 if single else 1e-14)  # atomics reorder sums only
+    for un in (1, 4):  # inner-loop unrolls of the symmetric kernel give the same sums
+        dev.set_sym_unroll(un)
+        assert rel_l2(dev.matvec(x).cpu().numpy(), y_sym) <= (1e-6 if single else 1e-14)
+    dev.set_sym_unroll(0)
+    if not single:
+        ref = BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(q).numpy()
+        if op is not ops.HELMHOLTZ_2D:
+            assert rel_l2(y_sym, ref) <= FP64_TOL

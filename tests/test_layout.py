@@ -143,3 +143,57 @@ def test_merged_layout_emulates_every_instance():
         off += b.N
     levels = [S.level for S in L.up]
     assert levels == sorted(set(levels), reverse=True)
+
+
+SYMMETRIC_BUILDS = [c for c in BUILDS if c[1].operator.kernel_id in (0, 1, 2, 3)] + [
+    ("helm3d", make_build(np.random.default_rng(2).random((2500, 3)), ops.HELMHOLTZ_3D, kappa=3.0, leaf_size=40), 3.0)]
+
+
+@pytest.mark.parametrize("case", SYMMETRIC_BUILDS, ids=lambda c: c[0])
+def test_symmetric_split_emulation_matches_one_sided(case):
+    """Pairs evaluated once in both directions plus the one-sided rest give the same apply."""
+    _, f, k = case
+    L = build_layout(f, max_sources=900)
+    assert L.split is not None
+    one, sym = L.split
+    assert one.interactions + 2 * sym.interactions == L.interactions
+    assert L.evaluations == one.interactions + sym.interactions < L.interactions
+    assert not np.any(sym.e_flags & 2), "self segments must stay one-sided"
+    assert np.all(sym.e_qkind == np.where(sym.e_q >= L.nslots, 1, 0))
+    q = np.random.RandomState(4).randn(f.N)
+    if f.operator.is_complex:
+        q = q + 1j * np.random.RandomState(5).randn(f.N)
+    y0 = emulate_apply(L, f.operator.name, k, q)
+    y1 = emulate_apply(L, f.operator.name, k, q, split=True)
+    assert rel_l2(y1, y0) < 1e-13
+
+
+def test_pair_rows_partition():
+    """Every row is one-sided, a pair representative, or the partner of exactly one representative."""
+    from skelfmm_amd.layout import pair_rows
+
+    R = np.array([
+        [0, 0, 1, 0, 0, 0], [1, 0, 0, 0, 0, 0],    # pair
+        [2, 1, 3, 1, 1, 1], [3, 1, 2, 1, 1, 1],    # pair (skeleton SUB)
+        [4, 1, 5, 0, 0, 1], [5, 0, 4, 1, 1, 1],    # pair (coarse up / down)
+        [6, 0, 6, 0, 0, 0],                        # self: one-sided
+        [7, 0, 8, 0, 0, 0],                        # partner missing: one-sided
+        [9, 0, 10, 0, 0, 0], [10, 0, 9, 0, 0, 1],  # signs differ: one-sided
+        [11, 0, 12, 0, 0, 0], [11, 0, 12, 0, 0, 0], [12, 0, 11, 0, 0, 0],  # duplicate: one-sided
+    ], dtype=np.int64)
+    one, rep = pair_rows(R, 13)
+    assert sorted(map(tuple, rep)) == [(0, 0, 1, 0, 0, 0), (2, 1, 3, 1, 1, 1), (4, 1, 5, 0, 0, 1)]
+    assert one.shape[0] + 2 * rep.shape[0] == R.shape[0]
+    assert {tuple(r) for r in one} == {tuple(r) for r in R[6:]}
+
+
+def test_merged_layout_split_emulates_every_instance():
+    from skelfmm_amd.layout import merge_layouts
+
+    rng = np.random.default_rng(9)
+    builds = [make_build(annulus_points(n, s), ops.LAPLACE_2D, tol=1e-5, leaf_size=40)
+              for n, s in ((1500, 1), (3000, 2))]
+    L = merge_layouts([build_layout(b, max_sources=600) for b in builds])
+    q = rng.normal(size=L.N)
+    np.testing.assert_allclose(emulate_apply(L, "laplace_2d", 0.0, q, split=True),
+                               emulate_apply(L, "laplace_2d", 0.0, q), rtol=0, atol=1e-11 * np.abs(q).sum())
