@@ -11,16 +11,20 @@ seconds are reported beside it.
 ``roofline`` is for the near-field kernel (the dominant one): algorithmic
 flops per launch (interactions x flops per interaction) over its mean launch
 duration, timed with HIP events on the stream it runs on.
-``cpu_baseline`` is the padded torch-CPU restatement of the reference apply on a
-sampled fraction of every interaction list plus full interpolation passes,
-extrapolated by interaction count.
+``cpu_baseline`` is the padded torch-CPU restatement of the reference apply run on
+two sampled fractions of every interaction list (with full interpolation passes) and
+extrapolated by a linear fit in the interaction count.
 
-Second workload (``--workload annuli``): 32 independent curvy annuli (2D Laplace,
+Other workloads: ``--workload annuli``: 32 independent curvy annuli (2D Laplace,
 N_i log-uniform in [1e4, 5e5], tol 1e-5, leaf 128) with 64 float32 right-hand
-sides each, applied together as one graph-captured apply.
+sides each, applied together as one graph-captured apply; ``sphere``: 1M points on
+the unit sphere, 3D Helmholtz kappa 10 pi, tol 1e-5, leaf 256 (``--double-layer``
+for the double layer), GPU build; ``mixture``: 2M points of a 16-component Gaussian
+mixture, 3D Helmholtz kappa 20, tol 1e-6, leaf 256, built twice on the GPU
+(adaptive IDs, then learned-rank sketched IDs), complex64 apply.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--no-cpu] [--no-tune]
-       python bench.py --workload annuli
+       python bench.py --workload annuli|sphere|mixture
 """
 
 from __future__ import annotations
@@ -46,10 +50,12 @@ FLOPS_PER_INTERACTION = {"laplace_3d": 11, "laplace_2d": 11,
                          # + (x - y).n (5) and the (ikr - 1) r^-2 factor (4)
                          "helmholtz_double_layer_3d": 32}
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
-# HBM bytes per launch of the FP32 near-field kernel on the default workload, from the two
+# HBM bytes per launch of the dominant FP32 near-field kernel on the default workload (one-sided
+# kernel, or the symmetric kernel: its FP64 reverse-sum atomics count as writes), from the two
 # rocprofv3 --pmc passes summarised in profiles/apply_laplace3d_uniform1M_hbm_traffic.md
 # (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, scale factors from a 512 MiB copy of the same run)
-P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400, False): 2549.1e6 + 14.2e6}
+P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400, False): 2549.1e6 + 14.2e6,
+                          (1_000_000, 1e-6, 400, True): 536.9e6 + 639.8e6}
 
 
 def log(msg):
@@ -277,6 +283,7 @@ def run_helmholtz(args):
         builds["learned"] = {"id_s": round(tid_l, 3), "lists_upload_s": round(tset_l, 2),
                              "build_s": round(t_tree + tid_l + tset_l, 2), "rank_sum": int(f_l.rank_list.sum()),
                              "stats": st, "fallback_rate": round(st["fallback"] / max(1, st["sketched"] + st["fallback"]), 4),
+                             "rejected_by": dict(f_l.gpu_skel.sketch_stats),
                              "level_s": {int(k): round(v, 3) for k, v in f_l.level_times.items()}}
         builds["id_speedup_learned_over_adaptive"] = round(tid_a / tid_l, 3)
         log(f"learned GPU build: IDs {tid_l:.2f}s ({tid_a / tid_l:.2f}x), stats {st}")
@@ -347,25 +354,73 @@ def run_helmholtz(args):
     print(json.dumps(line), flush=True)
 
 
+def cpu_apply_estimate(fmm, q, interactions, fraction):
+    """Seconds of one full CPU apply, from two sampled runs of the oracle's padded apply.
+
+    Runs on ``fraction`` and ``2 fraction`` of the pairs of every list (interpolation passes in
+    full both times) and fits t = a + b I over the sampled interaction counts I; the estimate is
+    a + b x ``interactions``. Returns (seconds, sample description)."""
+    from oracle.batched_apply import BatchedApplyCPU
+
+    pts = []
+    for f in (fraction, 2 * fraction):
+        ba = BatchedApplyCPU(fmm, bytes_budget=10e9, pair_fraction=f)
+        if not pts:
+            ba.matvec(q)  # warm-up
+        tc = time.perf_counter()
+        ba.matvec(q)
+        pts.append((ba.interactions, time.perf_counter() - tc))
+    (i1, t1), (i2, t2) = pts
+    b = max(t2 - t1, 0.0) / max(i2 - i1, 1)
+    a = max(t1 - b * i1, 0.0)
+    est = a + b * interactions
+    desc = (f"padded torch-CPU restatement of the reference apply on {fraction:.0%} and {2 * fraction:.0%} of the "
+            f"pairs of every interaction list ({i1:.3e} / {i2:.3e} interactions, {t1:.1f} / {t2:.1f} s) with full "
+            f"interpolation passes; linear fit t = a + b I (a = {a:.2f} s), evaluated at {interactions:.3e} "
+            f"interactions")
+    return est, desc
+
+
 def _cpu_baseline(fmm, q, interactions, N, args, fraction=0.01):
     """Padded torch-CPU restatement of the reference apply on a sample of every pair list."""
     import torch
 
-    from oracle.batched_apply import BatchedApplyCPU
-
     ncores = min(16, len(os.sched_getaffinity(0)))
     torch.set_num_threads(ncores)
-    ba = BatchedApplyCPU(fmm, bytes_budget=10e9, pair_fraction=fraction)
-    tc = time.perf_counter()
-    ba.matvec(q)
-    el = time.perf_counter() - tc
-    est = el * interactions / max(ba.interactions, 1)
+    est, desc = cpu_apply_estimate(fmm, q, interactions, fraction)
     cpu = {"value": round(N / est / 1e6, 6), "unit": "Mpoints*RHS/s", "cores": ncores, "kind": "port",
-           "sample": (f"padded torch-CPU complex128 restatement of the reference apply on {fraction:.0%} of the pairs "
-                      f"of every interaction list ({ba.interactions:.3e} interactions) plus full interpolation "
-                      f"passes, {el:.1f}s, extrapolated to {interactions:.3e} interactions")}
+           "sample": "complex128 " + desc}
     log(f"cpu baseline: {cpu}")
     return cpu
+
+
+def _p2p_roofline(d32, d64, s32, s64, st32, st64, p32, p64, tf32, tf64, fpi, N, args):
+    """Roofline of the dominant kernel: the symmetric near field when it runs, else the one-sided one."""
+    sym = bool(d32.use_symmetric and s32)
+    ach, ms = (st32, s32) if sym else (tf32, p32)
+    out = {
+        "bound": "mfma",
+        "achieved": round(ach, 3),
+        "peak": PEAK_TFLOPS["fp32"],
+        "unit": "TFLOP/s",
+        "frac": round(ach / PEAK_TFLOPS["fp32"], 4),
+        "traffic": P2P_FP32_TRAFFIC_BYTES.get((N, args.tol, args.leaf, sym)),
+        "traffic_unit": "bytes per launch (HBM read + write) of the dominant kernel",
+        "compute_unit": "VALU (per-pair kernel evaluation; no matrix cores at 1 RHS)",
+        "kernel": ("p2p_sym_kernel<float, LAPLACE_3D> (pairs of lists, both directions; zero fill and fold "
+                   "of the reverse sums included)" if sym else "p2p_kernel<float, LAPLACE_3D> (near field)"),
+        "kernel_ms": round(ms * 1e3, 4),
+        "flops_counted": (f"pairs x {fpi - ACC_FLOPS} + 2 pairs x {ACC_FLOPS} ({d32.sym_pairs} pairs)" if sym else
+                          f"interactions x {fpi}"),
+        "near_field_call": {"achieved": round(tf32, 3), "ms": round(p32 * 1e3, 4),
+                            "evaluations": int(d32.evaluations if d32.use_symmetric else d32.interactions),
+                            "interactions": int(d32.interactions)},
+        "fp64_kernel": {"achieved": round(st64 if (sym and st64) else tf64, 3), "peak": PEAK_TFLOPS["fp64"],
+                        "frac": round((st64 if (sym and st64) else tf64) / PEAK_TFLOPS["fp64"], 4),
+                        "kernel_ms": round((s64 if (sym and s64) else p64) * 1e3, 4),
+                        "near_field_call_ms": round(p64 * 1e3, 4)},
+    }
+    return out
 
 
 def main():
@@ -473,6 +528,9 @@ def main():
 
     p64 = p2p_time(d64)
     p32 = p2p_time(d32)
+    # dominant kernel alone: the symmetric near field (zero fill + p2p_sym_kernel + fold)
+    s32 = d32.time_near_field(1, reps=10, part="sym") * 1e-3 if d32.use_symmetric else None
+    s64 = d64.time_near_field(1, reps=10, part="sym") * 1e-3 if d64.use_symmetric else None
     fpi = FLOPS_PER_INTERACTION[op.name]
 
     def nf_flops(d):
@@ -481,6 +539,10 @@ def main():
 
     tf64 = nf_flops(d64) / p64 / 1e12
     tf32 = nf_flops(d32) / p32 / 1e12
+    # symmetric kernel: pairs x (flops - ACC_FLOPS) + 2 pairs x ACC_FLOPS
+    sym_flops = lambda d: d.sym_pairs * (fpi - ACC_FLOPS) + 2 * d.sym_pairs * ACC_FLOPS
+    st32 = sym_flops(d32) / s32 / 1e12 if s32 else None
+    st64 = sym_flops(d64) / s64 / 1e12 if s64 else None
     log(f"near-field kernel FP64 {p64 * 1e3:.2f} ms ({tf64:.1f} TF/s), FP32 {p32 * 1e3:.2f} ms ({tf32:.1f} TF/s)")
 
     # errors against the FP64 direct sum at sampled targets
@@ -496,28 +558,15 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        from oracle.batched_apply import BatchedApplyCPU
-
         ncores = min(16, len(os.sched_getaffinity(0)))
         torch.set_num_threads(ncores)
-        ba = BatchedApplyCPU(fmm, bytes_budget=10e9, pair_fraction=args.cpu_fraction)
-        ba.matvec(q[:])  # warm-up
-        best = None
-        for _ in range(2):
-            tc = time.perf_counter()
-            ba.matvec(q)
-            el = time.perf_counter() - tc
-            best = el if best is None else min(best, el)
-        frac_inter = ba.interactions
-        est = best * d64.interactions / max(frac_inter, 1)
+        est, desc = cpu_apply_estimate(fmm, q, d64.interactions, args.cpu_fraction)
         cpu = {
             "value": round(N / est / 1e6, 6),
             "unit": "Mpoints*RHS/s",
             "cores": ncores,
             "kind": "port",
-            "sample": (f"padded torch-CPU FP64 restatement of the reference apply on {args.cpu_fraction:.0%} of the "
-                       f"pairs of every interaction list ({frac_inter:.3e} interactions) plus full interpolation "
-                       f"passes, {best:.1f}s, extrapolated to {d64.interactions:.3e} interactions"),
+            "sample": "FP64 " + desc,
         }
         log(f"cpu baseline: {cpu}")
 
@@ -545,23 +594,7 @@ def main():
         "fp32_over_fp64_speedup": round(t64 / t32, 3),
         "build_s": round(build_s, 2),
         "build_split_s": {"tree": round(t_tree, 2), "skel": round(t_skel, 2), "lists_upload": round(t_setup, 2)},
-        "roofline": {
-            "bound": "mfma",
-            "achieved": round(tf32, 3),
-            "peak": PEAK_TFLOPS["fp32"],
-            "unit": "TFLOP/s",
-            "frac": round(tf32 / PEAK_TFLOPS["fp32"], 4),
-            "traffic": P2P_FP32_TRAFFIC_BYTES.get((N, args.tol, args.leaf, bool(d32.use_symmetric))),
-            "traffic_unit": "bytes per launch (HBM read + write)",
-            "compute_unit": "VALU (per-pair kernel evaluation; no matrix cores at 1 RHS)",
-            "kernel": ("near field: p2p_sym_kernel<float, LAPLACE_3D> (pairs, both directions) + p2p_kernel "
-                       "(self segments)" if d32.use_symmetric else "p2p_kernel<float, LAPLACE_3D> (near field)"),
-            "kernel_ms": round(p32 * 1e3, 4),
-            "flops_per_interaction": fpi,
-            "kernel_evaluations": int(d32.evaluations if d32.use_symmetric else d32.interactions),
-            "fp64_kernel": {"achieved": round(tf64, 3), "peak": PEAK_TFLOPS["fp64"],
-                            "frac": round(tf64 / PEAK_TFLOPS["fp64"], 4), "kernel_ms": round(p64 * 1e3, 4)},
-        },
+        "roofline": _p2p_roofline(d32, d64, s32, s64, st32, st64, p32, p64, tf32, tf64, fpi, N, args),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -151,6 +151,9 @@ class SkelFMMBatched(torch.nn.Module):
         self.sym_unroll = 0
         self.interactions = L.interactions
         self.evaluations = getattr(L, "evaluations", L.interactions)
+        # pairs evaluated by the symmetric kernel / interactions of the one-sided rest
+        self.sym_pairs = L.split[1].interactions if self._split is not None else 0
+        self.one_interactions = L.split[0].interactions if self._split is not None else L.interactions
         self.nslots = L.nslots
         self.N = L.N
         for k in list(self._plans):
@@ -308,10 +311,23 @@ class SkelFMMBatched(torch.nn.Module):
             self._destroy_plan(k)
         self._graphs = {}
 
-    def time_near_field(self, width, reps=5):
-        """Mean duration (ms) of one near-field launch for ``width`` channels, HIP events on the current stream."""
+    def time_near_field(self, width, reps=5, part="all"):
+        """Mean duration (ms) of one near-field launch for ``width`` channels, HIP events on the current stream.
+
+        ``part="sym"`` times the call with the one-sided rest emptied (zero fill, symmetric
+        kernel, fold of the reverse sums), ``part="one"`` the rest alone; both only when the
+        symmetric split is in use."""
         p = self._plan(width)
         a = self._p2p_args(width, p["urev"])
+        if part != "all":
+            if not (self.use_symmetric and a.sym):
+                raise ValueError("near-field parts exist only with the symmetric split in use")
+            empty = hip.P2PLists()
+            keep = self._split_structs[1] if part == "sym" else self._split_structs[0]
+            if part == "sym":
+                a.one, a.sym = ctypes.pointer(empty), ctypes.pointer(keep)
+            else:
+                a.one, a.sym = ctypes.pointer(keep), ctypes.pointer(empty)
         a.q = hip.ptr(p["qbuf"])
         a.u = hip.ptr(p["ubuf"])
         st = hip.stream_handle(self.device)

@@ -1001,6 +1001,128 @@ __global__ __launch_bounds__(256) void up_gemv_kernel(const R* __restrict__ T, c
   }
 }
 
+// One channel (one real or complex right-hand side): each wave takes its 4 rows of the item
+// together, so every load of the shared vector slice serves 4 rows and 5 loads are in flight per
+// step; one wave reduction per row at the end.
+template <typename R, bool CX>
+__global__ __launch_bounds__(256) void up_gemv1_kernel(const R* __restrict__ T, const int* __restrict__ box_off,
+                                                       const int* __restrict__ box_bs, const int* __restrict__ box_rank,
+                                                       const long long* __restrict__ box_toff,
+                                                       const int2* __restrict__ items, const R* __restrict__ qvec,
+                                                       R* __restrict__ qskel, int w) {
+  constexpr int CWd = CX ? 2 : 1;
+  const int2 it = items[blockIdx.x];
+  const int b = it.x;
+  const int off = box_off[b], bs = box_bs[b], r = box_rank[b];
+  const long long toff = box_toff[b];
+  const int nred = bs - r;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j0 = it.y + wave * kRowsPerWave;  // wave-uniform
+  if (j0 >= r) return;
+  R ar[kRowsPerWave], ai[kRowsPerWave];
+  const R* Trow[kRowsPerWave];
+#pragma unroll
+  for (int rr = 0; rr < kRowsPerWave; ++rr) {
+    ar[rr] = ai[rr] = 0;
+    Trow[rr] = T + (toff + (long long)min(j0 + rr, r - 1) * nred) * CWd;
+  }
+  const R* x = qvec + (long long)(off + r) * w;
+#pragma unroll 2
+  for (int i = lane; i < nred; i += 64) {
+    const R xr = x[(long long)i * w];
+    const R xi = CX ? x[(long long)i * w + 1] : R(0);
+#pragma unroll
+    for (int rr = 0; rr < kRowsPerWave; ++rr) {
+      const R tr = Trow[rr][(long long)i * CWd];
+      const R ti = CX ? Trow[rr][(long long)i * CWd + 1] : R(0);
+      cmac<R, CX>(ar[rr], ai[rr], tr, ti, xr, xi);
+    }
+  }
+#pragma unroll
+  for (int rr = 0; rr < kRowsPerWave; ++rr) {
+    for (int o = 32; o > 0; o >>= 1) {
+      ar[rr] += __shfl_xor(ar[rr], o, 64);
+      if (CX) ai[rr] += __shfl_xor(ai[rr], o, 64);
+    }
+  }
+  if (lane < kRowsPerWave) {
+    // lane rr writes row j0 + rr (static register selection below)
+    R vr = ar[0], vi = ai[0];
+#pragma unroll
+    for (int rr = 1; rr < kRowsPerWave; ++rr)
+      if (lane == rr) {
+        vr = ar[rr];
+        vi = ai[rr];
+      }
+    const int j = j0 + lane;
+    if (j < r) {
+      const R* xs = qvec + (long long)(off + j) * w;
+      R* y = qskel + (long long)(off + j) * w;
+      y[0] = xs[0] + vr;
+      if (CX) y[1] = xs[1] + vi;
+    }
+  }
+}
+
+// One channel, downward: thread per column, rows unrolled by 8 (8 independent loads in flight);
+// the skeleton potentials of a chunk of rows are staged in LDS and read as broadcasts.
+template <typename R, bool CX>
+__global__ __launch_bounds__(256) void down_gemv1_kernel(const R* __restrict__ T, const int* __restrict__ box_off,
+                                                         const int* __restrict__ box_bs,
+                                                         const int* __restrict__ box_rank,
+                                                         const long long* __restrict__ box_toff,
+                                                         const int2* __restrict__ items, R* __restrict__ uvec,
+                                                         const R* __restrict__ uskel, int w) {
+  constexpr int CWd = CX ? 2 : 1;
+  constexpr int JB = 256;
+  __shared__ R us[JB * CWd];
+  const int2 it = items[blockIdx.x];
+  const int b = it.x;
+  const int off = box_off[b], bs = box_bs[b], r = box_rank[b];
+  const long long toff = box_toff[b];
+  const int nred = bs - r;
+  const int i = it.y + threadIdx.x;
+  const bool act = i < nred;
+  R ar0 = 0, ai0 = 0, ar1 = 0, ai1 = 0;
+  const R* tcol = T + (toff + (act ? i : 0)) * CWd;
+  for (int j0 = 0; j0 < r; j0 += JB) {
+    const int nj = min(JB, r - j0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nj; t += blockDim.x) {
+      const R* xs = uskel + (long long)(off + j0 + t) * w;
+      us[t * CWd] = xs[0];
+      if (CX) us[t * CWd + 1] = xs[1];
+    }
+    __syncthreads();
+    if (act) {
+      int jj = 0;
+      for (; jj + 8 <= nj; jj += 8) {
+        R tr[8], ti[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const R* tp = tcol + (long long)(j0 + jj + k) * nred * CWd;
+          tr[k] = tp[0];
+          ti[k] = CX ? tp[1] : R(0);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k += 2) {
+          cmac<R, CX>(ar0, ai0, tr[k], ti[k], us[(jj + k) * CWd], CX ? us[(jj + k) * CWd + 1] : R(0));
+          cmac<R, CX>(ar1, ai1, tr[k + 1], ti[k + 1], us[(jj + k + 1) * CWd], CX ? us[(jj + k + 1) * CWd + 1] : R(0));
+        }
+      }
+      for (; jj < nj; ++jj) {
+        const R* tp = tcol + (long long)(j0 + jj) * nred * CWd;
+        cmac<R, CX>(ar0, ai0, tp[0], CX ? tp[1] : R(0), us[jj * CWd], CX ? us[jj * CWd + 1] : R(0));
+      }
+    }
+  }
+  if (act) {
+    R* y = uvec + (long long)(off + r + i) * w;
+    y[0] += ar0 + ar1;
+    if (CX) y[1] += ai0 + ai1;
+  }
+}
+
 // Downward: u[off+r+i] += sum_j T[j,i] uskel[off+j] for the item's columns i (thread per column).
 template <typename R, bool CX>
 __global__ __launch_bounds__(256) void down_gemv_kernel(const R* __restrict__ T, const int* __restrict__ box_off,
@@ -1188,6 +1310,13 @@ int run_up_level(const skf_level_args* L, const void* T, int cx, void* qvec, voi
       else
         hipLaunchKernelGGL((up_gemm_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
                            L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
+    } else if (nchan == 1) {
+      if (cx)
+        hipLaunchKernelGGL((up_gemv1_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
+      else
+        hipLaunchKernelGGL((up_gemv1_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
     } else if (cx)
       hipLaunchKernelGGL((up_gemv_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
                          L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
@@ -1217,6 +1346,13 @@ int run_down_level(const skf_level_args* L, const void* T, int cx, void* uvec, v
                            L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
       else
         hipLaunchKernelGGL((down_gemm_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
+    } else if (nchan == 1) {
+      if (cx)
+        hipLaunchKernelGGL((down_gemv1_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
+                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
+      else
+        hipLaunchKernelGGL((down_gemv1_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
                            L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
     } else if (cx)
       hipLaunchKernelGGL((down_gemv_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),

@@ -93,6 +93,7 @@ class GPUSkeletonizer:
         self.gen = torch.Generator(device=self.device)
         self.gen.manual_seed(seed)
         self.timings = {}
+        self.sketch_stats = {}  # rejected sketches by reason: rows_exhausted (rank too low), residual
         self.profile = False  # True: synchronize and time every phase (proxy, gemm, qr, check, host)
         self._t = None
 
@@ -341,7 +342,12 @@ class GPUSkeletonizer:
             res_h, zn_h = res.cpu().numpy(), zn.cpu().numpy()
             self._phase("check")
             cap = np.minimum(m, ncb)
-            good = host["conv"] & ((host["rank"] < lg) | (lg >= cap)) & (res_h <= accept * self.tol * np.maximum(zn_h, 1e-300))
+            rows_ok = host["conv"] & ((host["rank"] < lg) | (lg >= cap))
+            res_ok = res_h <= accept * self.tol * np.maximum(zn_h, 1e-300)
+            good = rows_ok & res_ok
+            st = self.sketch_stats
+            st["rows_exhausted"] = st.get("rows_exhausted", 0) + int(np.sum(~rows_ok))
+            st["residual"] = st.get("residual", 0) + int(np.sum(rows_ok & ~res_ok))
             for i, r in zip(boxes, self._results(d, host, good)):
                 results[i] = r
             del Ybuf, Zbuf, d

@@ -244,7 +244,7 @@ def harvest(points, operator, kappa, tol, leaf_size, geometry=None, npoints_max=
     return np.concatenate(Fs), np.concatenate(rs).astype(np.int64)
 
 
-def _cloud(kind, n, dim, rng):
+def _cloud(kind, n, dim, rng, ncomp=16, sigma=0.02):
     if kind == "cube":
         return rng.random((n, dim)), None
     if kind == "sphere":
@@ -252,10 +252,10 @@ def _cloud(kind, n, dim, rng):
         x = v / np.linalg.norm(v, axis=1, keepdims=True)
         return x, x.copy()
     if kind == "mixture":
-        c = rng.random((16, dim))
-        w = rng.dirichlet(np.ones(16))
-        comp = rng.choice(16, size=n, p=w)
-        x = np.clip(c[comp] + 0.02 * rng.normal(size=(n, dim)), 0.0, 1.0)
+        c = rng.random((ncomp, dim))
+        w = rng.dirichlet(np.ones(ncomp))
+        comp = rng.choice(ncomp, size=n, p=w)
+        x = np.clip(c[comp] + sigma * rng.normal(size=(n, dim)), 0.0, 1.0)
         return x, None
     if kind == "annulus":
         th = rng.random(n) * 2 * np.pi
@@ -264,17 +264,21 @@ def _cloud(kind, n, dim, rng):
     raise ValueError(kind)
 
 
-def harvest_suite(n_builds=48, seed=0, max_points=60000, verbose=False):
+def harvest_suite(n_builds=48, seed=0, max_points=60000, verbose=False, suite="mixed"):
     """Pairs from host builds over uniform-cube, sphere-surface and mixture clouds.
 
-    The committed set ``data/rank_pairs.npz`` (196,075 pairs from 104 builds) is
-    ``harvest --builds 72 --seed 0 --max-points 80000`` followed by
-    ``harvest --builds 32 --seed 1 --max-points 80000 --append``.
+    ``suite="mixed"``: operators, wavenumbers in [0, 40], tolerances in {1e-4, ..., 1e-8}
+    and leaf sizes drawn per build from ``default_rng(seed)``. ``suite="clustered"``:
+    large 3D Gaussian mixtures (8-32 components, sigma 0.01-0.05), Helmholtz or Laplace,
+    tol 1e-5..1e-7, leaf 128 or 256 -- deep adaptive trees.
 
-    Operators, wavenumbers in [0, 40], tolerances in {1e-4, ..., 1e-8} and leaf
-    sizes are drawn per build from ``default_rng(seed)``.
+    The committed set ``data/rank_pairs.npz`` is ``harvest --builds 72 --seed 0
+    --max-points 80000``, then ``--builds 32 --seed 1 --max-points 80000 --append``,
+    then ``--suite clustered --builds 16 --seed 2 --max-points 600000 --append``.
     """
     O = operators_mod
+    if suite == "clustered":
+        return _harvest_clustered(n_builds, seed, max_points, verbose)
     choices_3d = [O.LAPLACE_3D, O.HELMHOLTZ_3D, O.HELMHOLTZ_3D, O.LAPLACE_DOUBLE_LAYER_3D,
                   O.HELMHOLTZ_DOUBLE_LAYER_3D, O.LAPLACE_ADJOINT_DOUBLE_LAYER_3D,
                   O.HELMHOLTZ_ADJOINT_DOUBLE_LAYER_3D]
@@ -308,6 +312,29 @@ def harvest_suite(n_builds=48, seed=0, max_points=60000, verbose=False):
     return np.concatenate(Fs), np.concatenate(rs), meta
 
 
+def _harvest_clustered(n_builds, seed, max_points, verbose):
+    O = operators_mod
+    rng = np.random.default_rng(seed)
+    Fs, rs, meta = [], [], []
+    for b in range(n_builds):
+        op = O.HELMHOLTZ_3D if rng.random() < 0.75 else O.LAPLACE_3D
+        kappa = float(rng.uniform(0, 40)) if op is O.HELMHOLTZ_3D else 0.0
+        tol = float(10.0 ** -rng.integers(5, 8))
+        leaf = int(rng.choice([128, 256]))
+        n = int(rng.integers(max_points // 4, max_points + 1))
+        ncomp = int(rng.choice([8, 16, 32]))
+        sigma = float(rng.choice([0.01, 0.02, 0.05]))
+        x, _ = _cloud("mixture", n, 3, rng, ncomp=ncomp, sigma=sigma)
+        F, r = harvest(x, op, kappa, tol, leaf)
+        Fs.append(F)
+        rs.append(r)
+        meta.append((op.name, f"mixture{ncomp}/{sigma}", n, kappa, tol, leaf, F.shape[0]))
+        if verbose:
+            print(f"build {b:3d}: {op.name:14s} mixture {ncomp:2d}x{sigma:.2f} N={n:7d} kappa={kappa:5.1f} "
+                  f"tol={tol:.0e} leaf={leaf:3d} -> {F.shape[0]} pairs", flush=True)
+    return np.concatenate(Fs), np.concatenate(rs), meta
+
+
 def _main(argv=None):
     import argparse
 
@@ -319,13 +346,14 @@ def _main(argv=None):
     h.add_argument("--max-points", type=int, default=60000)
     h.add_argument("--out", default=default_pairs_path())
     h.add_argument("--append", action="store_true", help="add to the pairs already in --out")
+    h.add_argument("--suite", choices=("mixed", "clustered"), default="mixed")
     t = sub.add_parser("train")
     t.add_argument("--pairs", default=default_pairs_path())
     t.add_argument("--out", default=default_model_path())
     t.add_argument("--epochs", type=int, default=60)
     a = ap.parse_args(argv)
     if a.cmd == "harvest":
-        F, r, meta = harvest_suite(a.builds, a.seed, a.max_points, verbose=True)
+        F, r, meta = harvest_suite(a.builds, a.seed, a.max_points, verbose=True, suite=a.suite)
         F, r = F.astype(np.float32), r.astype(np.int32)
         builds = np.array([" ".join(map(str, m)) for m in meta])
         if a.append and os.path.exists(a.out):
