@@ -592,7 +592,8 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   constexpr int DIM = S::dim;
   constexpr int NG = S::ng;
   __shared__ R qt[64 * kQS];
-  __shared__ R pt[64 * 8];  // x, y, z, nx, ny, nz per source
+  __shared__ R pt[64 * 8];   // x, y, z, nx, ny, nz per source (precision form: in the target box frame)
+  __shared__ int pself[64];  // local index of a self-segment source (its diagonal target), else -1
   const int4 it = d.items[blockIdx.x];
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
   const int c0 = blockIdx.y * kMB;
@@ -601,6 +602,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ng = d.g_n[g];
+  const bool wave_busy = t0 + wave * 16 < ng;     // waves without targets only stage
   const int tloc = t0 + wave * 16 + (lane & 15);  // target of this lane's A rows
   const bool tvalid = tloc < ng;
   const int tslot = d.g_pt[g] + (tvalid ? tloc : t0);
@@ -608,7 +610,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   const R* py = static_cast<const R*>(d.py);
   const R* pz = static_cast<const R*>(d.pz);
   const R* qv = static_cast<const R*>(d.q);
-  R xt = px[tslot], yt = py[tslot], zt = (DIM == 3) ? pz[tslot] : R(0);
+  const R xt = px[tslot], yt = py[tslot], zt = (DIM == 3) ? pz[tslot] : R(0);
   R gxt = 0, gyt = 0, gzt = 0;
   if (OpT<OP>::geom == 2) {
     gxt = static_cast<const R*>(d.nx)[tslot];
@@ -628,54 +630,71 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) acc[nb][r] = 0.0;
 
-  for (int s = sb; s < se; ++s) {
-    const int ps = d.e_pt[s], ns = d.e_n[s], qo = d.e_q[s], fl = d.e_flags[s];
-    const bool self = (fl & SKF_SRC_SELF) != 0;
-    const double sgn = (fl & SKF_SRC_NEG) ? -1.0 : 1.0;
-    R xs0 = xt, ys0 = yt, zs0 = zt;
-    if (PREC) {
-      const int eb = d.e_box[s];
-      xs0 = xt + R(ctx - d.centers[3 * eb + 0]);
-      ys0 = yt + R(cty - d.centers[3 * eb + 1]);
-      zs0 = zt + R(ctz - d.centers[3 * eb + 2]);
-    }
-    for (int base = 0; base < ns; base += 64) {
-      const int nsrc = min(64, ns - base);
-      // stage 64 sources: coordinates (threads 0..63) and the 64 x 64 charge block (all threads)
-      if (tid < 64) {
-        const int j = base + tid;
-        R* row = &pt[tid * 8];
-        if (tid < nsrc) {
-          const int sl = ps + j;
-          row[0] = px[sl];
-          row[1] = py[sl];
-          row[2] = (DIM == 3) ? pz[sl] : R(0);
+  // The item's entries form one stream of sources; tiles of 64 are cut from it across entry
+  // boundaries (small skeleton entries fill a tile together). Each staged source carries its
+  // entry's sign (folded into its charges), self flag and, in the precision form, the shift
+  // into the target box's frame.
+  int tot = 0;
+  for (int s = sb; s < se; ++s) tot += d.e_n[s];
+  int cs = sb, cj = 0;  // stream cursor of the tile start (uniform)
+  for (int base = 0; base < tot; base += 64) {
+    const int nsrc = min(64, tot - base);
+    {
+      const int sr = tid >> 2, cq = (tid & 3) * 16;
+      R* dst = &qt[sr * kQS + cq];
+      if (sr < nsrc) {
+        int s = cs, j = cj + sr;
+        while (j >= d.e_n[s]) {
+          j -= d.e_n[s];
+          ++s;
+        }
+        const int fl = d.e_flags[s];
+        const R sgn = (fl & SKF_SRC_NEG) ? R(-1) : R(1);
+        const R* src = qv + (long long)(d.e_q[s] + j) * d.width + c0 + cq;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) dst[c] = (cq + c < ncv) ? sgn * src[c] : R(0);
+        if ((tid & 3) == 0) {
+          const int sl = d.e_pt[s] + j;
+          R* row = &pt[sr * 8];
+          R ox = 0, oy = 0, oz = 0;
+          if (PREC) {
+            const int eb = d.e_box[s];
+            ox = R(ctx - d.centers[3 * eb + 0]);
+            oy = R(cty - d.centers[3 * eb + 1]);
+            oz = R(ctz - d.centers[3 * eb + 2]);
+          }
+          row[0] = px[sl] - ox;
+          row[1] = py[sl] - oy;
+          row[2] = (DIM == 3) ? pz[sl] - oz : R(0);
           if (NG) {
             row[3] = static_cast<const R*>(d.nx)[sl];
             row[4] = static_cast<const R*>(d.ny)[sl];
             row[5] = static_cast<const R*>(d.nz)[sl];
           }
-        } else {
+          pself[sr] = (fl & SKF_SRC_SELF) ? j : -1;
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) dst[c] = R(0);
+        if ((tid & 3) == 0) {
           // far away, zero charge: contributes exactly zero
+          R* row = &pt[sr * 8];
           row[0] = R(1e18);
           row[1] = R(1e18);
           row[2] = R(1e18);
           row[3] = row[4] = row[5] = R(0);
+          pself[sr] = -1;
         }
       }
-      {
-        const int sr = tid >> 2, cq = (tid & 3) * 16;
-        R* dst = &qt[sr * kQS + cq];
-        if (sr < nsrc) {
-          const R* src = qv + (long long)(qo + base + sr) * d.width + c0 + cq;
-#pragma unroll
-          for (int c = 0; c < 16; ++c) dst[c] = (cq + c < ncv) ? src[c] : R(0);
-        } else {
-#pragma unroll
-          for (int c = 0; c < 16; ++c) dst[c] = R(0);
-        }
-      }
-      __syncthreads();
+    }
+    // advance the cursor by the tile (uniform)
+    cj += nsrc;
+    while (cs < se && cj >= d.e_n[cs]) {
+      cj -= d.e_n[cs];
+      ++cs;
+    }
+    __syncthreads();
+    if (wave_busy) {
       V D[4];
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) D[nb] = V{0, 0, 0, 0};
@@ -683,12 +702,12 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
       for (int kk = 0; kk < ksteps; ++kk) {
         const int sj = kk * 4 + (lane >> 4);
         const R* row = &pt[sj * 8];
-        const R dx = xs0 - row[0], dy = ys0 - row[1], dz = (DIM == 3) ? zs0 - row[2] : R(0);
+        const R dx = xt - row[0], dy = yt - row[1], dz = (DIM == 3) ? zt - row[2] : R(0);
         R rn = 0;
         if (OpT<OP>::geom == 1) rn = dx * row[3] + dy * row[4] + dz * row[5];
         if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
         R a = kernel_value<R, OP>(dx, dy, dz, rn, R(0)).re;
-        if (self && (base + sj) == tloc) a = R(0);
+        if (pself[sj] == tloc) a = R(0);
         const R* qrow = &qt[sj * kQS + (lane & 15)];
 #pragma unroll
         for (int nb = 0; nb < 4; ++nb)
@@ -697,9 +716,9 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) acc[nb][r] += sgn * double(D[nb][r]);
-      __syncthreads();
+        for (int r = 0; r < 4; ++r) acc[nb][r] += double(D[nb][r]);
     }
+    __syncthreads();
   }
   R* uo = static_cast<R*>(d.u);
 #pragma unroll

@@ -325,7 +325,12 @@ def _harvest_clustered(n_builds, seed, max_points, verbose):
         ncomp = int(rng.choice([8, 16, 32]))
         sigma = float(rng.choice([0.01, 0.02, 0.05]))
         x, _ = _cloud("mixture", n, 3, rng, ncomp=ncomp, sigma=sigma)
-        F, r = harvest(x, op, kappa, tol, leaf)
+        try:
+            F, r = harvest(x, op, kappa, tol, leaf)
+        except ValueError as e:  # e.g. clipped points piling up on a cube corner beyond leaf_size
+            if verbose:
+                print(f"build {b:3d}: skipped ({e})", flush=True)
+            continue
         Fs.append(F)
         rs.append(r)
         meta.append((op.name, f"mixture{ncomp}/{sigma}", n, kappa, tol, leaf, F.shape[0]))
