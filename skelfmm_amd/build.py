@@ -220,10 +220,13 @@ class RecursiveSkeletonization:
             F = box_features(self, lev, boxes, centers, length)
             k_pred = rank_model.predict_rank(F)
             res = gpu.sketched(cands, centers, length, nproxy, k_pred, **self._sketch_opts)
-            self.gpu_stats["sketched"] += sum(r is not None and not isinstance(r, str) for r in res)
+            kept = [isinstance(r, tuple) and len(r) == 2 and r[0] == "kept" for r in res]
+            sk = [r is not None and not isinstance(r, str) and not k for r, k in zip(res, kept)]
+            self.gpu_stats["kept"] += sum(kept)
+            self.gpu_stats["sketched"] += sum(sk)
             self.gpu_stats["direct"] += sum(isinstance(r, str) for r in res)
             self.gpu_stats["fallback"] += sum(r is None for r in res)
-            res = [None if isinstance(r, str) else r for r in res]
+            res = [r[1] if k else (None if isinstance(r, str) else r) for r, k in zip(res, kept)]
         todo = [i for i, r in enumerate(res) if r is None]
         if todo:
             sub = gpu.adaptive([cands[i] for i in todo], centers[todo], length, nproxy)
@@ -259,8 +262,9 @@ class RecursiveSkeletonization:
         self.root_level = max(0, self.tree.nlevels - 1)
         self.build_device = device
         # boxes by path: accepted sketches, sketches rejected (fallback), predicted incompressible
-        # (direct), adaptive GPU IDs (fallback + direct + no model), host IDs (LDS too small)
-        self.gpu_stats = {"sketched": 0, "fallback": 0, "direct": 0, "adaptive": 0, "cpu": 0}
+        # (direct), predicted full rank and kept whole (kept), adaptive GPU IDs (fallback + direct
+        # + no model), host IDs (LDS too small)
+        self.gpu_stats = {"sketched": 0, "fallback": 0, "direct": 0, "kept": 0, "adaptive": 0, "cpu": 0}
         gpu = None
         if device == "gpu":
             from .gpu_build import GPUSkeletonizer

@@ -74,6 +74,7 @@ struct Mth<double> {
     return fma(y * e, fma(0.375, e, 0.5), y);
   }
   __device__ static inline double lg(double x) { return log(x); }
+  __device__ static inline double half_lg(double x) { return 0.5 * log(x); }
   __device__ static inline void sc(double x, double* s, double* c) { sincos(x, s, c); }
   __device__ static inline double bj0(double x) { return j0(x); }
   __device__ static inline double by0(double x) { return y0(x); }
@@ -85,7 +86,9 @@ struct Mth<float> {
   static constexpr float clamp_cd = 1e-20f;
   static constexpr float clamp_dl = 1e-20f;
   __device__ static inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
-  __device__ static inline float lg(float x) { return __logf(x); }
+  // natural log from the hardware log2 (x >= 1e-20 here: normal, no denormal scaling needed)
+  __device__ static inline float lg(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
+  __device__ static inline float half_lg(float x) { return __builtin_amdgcn_logf(x) * 0.34657359027997265f; }
   __device__ static inline void sc(float x, float* s, float* c) { sincosf(x, s, c); }
   __device__ static inline float bj0(float x) { return j0f(x); }
   __device__ static inline float by0(float x) { return y0f(x); }
@@ -120,7 +123,7 @@ __device__ inline Cplx<R> kernel_value(R dx, R dy, R dz, R rn, R kappa) {
     if (OpT<OP>::dim == 3) r2 += dz * dz;
   }
   if (OP == SKF_LAPLACE_2D) {
-    k.re = R(0.5) * M::lg(M::fp32_bias ? r2 : fmax(r2, M::clamp_cd));
+    k.re = M::half_lg(M::fp32_bias ? r2 : fmax(r2, M::clamp_cd));
     k.im = 0;
   } else if (OP == SKF_LAPLACE_3D) {
     k.re = M::rsq(M::fp32_bias ? r2 : fmax(r2, M::clamp_cd));
@@ -389,25 +392,19 @@ __device__ inline void wave_sync_lds() {
   asm volatile("" ::: "memory");
 }
 
-template <typename R, int OP, bool PREC, int UNR>
-__global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
+// One pass of the symmetric kernel over targets tp .. tp + 64 TPL - 1 of the item (TPL per lane).
+template <typename R, int OP, bool PREC, int UNR, int TPL>
+__device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, R* __restrict__ rvl,
+                                         double* __restrict__ red, int tp, int tend, int t0, int sb, int se,
+                                         double ctx, double cty, double ctz) {
   using S = Stage<OP, 1>;
   constexpr int NV = S::nv;
   constexpr int CW = S::cw;
-  constexpr int TPL = SymTPL<R>::n;
   constexpr int SUB = SymSub<R, CW>::n;
   constexpr int LPS = SymSub<R, CW>::lanes;
-  static_assert(OpT<OP>::geom == 0, "symmetric evaluation needs a single-layer kernel");
-  static_assert(kSymTargets % (64 * TPL) == 0, "passes must tile a work item");
-  __shared__ R tile[kWG * NV];
-  __shared__ R rvl[4][SUB][CW][64];   // reverse partials of one block of sources, per wave
-  __shared__ double red[4][kTCH][CW];
-
-  const int4 it = d.items[blockIdx.x];
-  const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int ng = d.g_n[g];
+  const int g = reinterpret_cast<const int4*>(d.items)[blockIdx.x].x;
   const int gpt = d.g_pt[g], gout = d.g_out[g];
   const R* px = static_cast<const R*>(d.px);
   const R* py = static_cast<const R*>(d.py);
@@ -415,6 +412,145 @@ __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
   const R* qv = static_cast<const R*>(d.q);
   R* uo = static_cast<R*>(d.u);
   const int w = d.width;
+  const R kappa = R(d.kappa);
+  // targets tp + lane + 64 k; padding targets repeat the first one with zero charge
+  R qt[TPL * CW], xt[TPL], yt[TPL], zt[TPL];
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+    const int t = tp + lane + 64 * k;
+    const int sl = gpt + (t < tend ? t : t0);
+    xt[k] = px[sl];
+    yt[k] = py[sl];
+    zt[k] = (S::dim == 3) ? pz[sl] : R(0);
+#pragma unroll
+    for (int c = 0; c < CW; ++c) qt[k * CW + c] = (t < tend) ? qv[(long long)(gout + t) * w + c] : R(0);
+  }
+  double acc[TPL * CW];
+#pragma unroll
+  for (int a = 0; a < TPL * CW; ++a) acc[a] = 0.0;
+
+  for (int s = sb; s < se; ++s) {
+    const int ps = d.e_pt[s], ns = d.e_n[s], qo = d.e_q[s], fl = d.e_flags[s];
+    const double sgn = (fl & SKF_SRC_NEG) ? -1.0 : 1.0;
+    // precision form: sources are shifted into the target box's frame (minus the difference of
+    // the centres), one shift per source for all of a lane's targets
+    R ox = 0, oy = 0, oz = 0;
+    if (PREC) {
+      const int eb = d.e_box[s];
+      ox = R(ctx - d.centers[3 * eb + 0]);
+      oy = R(cty - d.centers[3 * eb + 1]);
+      oz = R(ctz - d.centers[3 * eb + 2]);
+    }
+    for (int base = 0; base < ns; base += kWG) {
+      {
+        const int j = base + tid;
+        R* row = &tile[tid * NV];
+        if (j < ns) {
+          const int sl = ps + j;
+          row[0] = px[sl];
+          row[1] = py[sl];
+          if (S::dim == 3) row[2] = pz[sl];
+          const R* qs = qv + (long long)(qo + j) * w;
+#pragma unroll
+          for (int c = 0; c < CW; ++c) row[S::dim + c] = qs[c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < NV; ++c) row[c] = R(0);
+        }
+      }
+      __syncthreads();
+      const int wb = base + wave * 64;  // wave-uniform
+      const int cnt = __builtin_amdgcn_readfirstlane(max(0, min(64, ns - wb)));
+      const R* tw = &tile[wave * 64 * NV];
+      R part[TPL * CW];
+#pragma unroll
+      for (int a = 0; a < TPL * CW; ++a) part[a] = 0;
+      for (int sub = 0; sub < cnt; sub += SUB) {
+        // only the block's valid rows are evaluated; table rows past them are never summed
+        const int nj = min(SUB, cnt - sub);
+#pragma unroll UNR
+        for (int jj = 0; jj < nj; ++jj) {
+          const R* row = tw + (sub + jj) * NV;
+          const R xs = PREC ? row[0] - ox : row[0];
+          const R ys = PREC ? row[1] - oy : row[1];
+          const R zs = (S::dim == 3) ? (PREC ? row[2] - oz : row[2]) : R(0);
+          R r0 = 0, r1 = 0;
+#pragma unroll
+          for (int k = 0; k < TPL; ++k) {
+            const R dx = xt[k] - xs;
+            const R dy = yt[k] - ys;
+            const R dz = (S::dim == 3) ? zt[k] - zs : R(0);
+            const Cplx<R> kv = kernel_value<R, OP>(dx, dy, dz, R(0), kappa);
+            if (CW == 1) {
+              part[k] += kv.re * row[S::dim];
+              r0 += kv.re * qt[k];
+            } else {
+              const R sr = row[S::dim], si = row[S::dim + 1];
+              part[2 * k] += kv.re * sr - kv.im * si;
+              part[2 * k + 1] += kv.re * si + kv.im * sr;
+              const R tr = qt[2 * k], ti = qt[2 * k + 1];
+              r0 += kv.re * tr - kv.im * ti;
+              r1 += kv.re * ti + kv.im * tr;
+            }
+          }
+          rvl[((wave * SUB + jj) * CW + 0) * 64 + lane] = r0;
+          if (CW == 2) rvl[((wave * SUB + jj) * CW + CW - 1) * 64 + lane] = r1;
+        }
+        wave_sync_lds();
+        // column sums: lanes j * LPS .. j * LPS + LPS - 1 hold source j's partial over 64 / LPS lanes
+        const int j = lane / LPS, pp = lane % LPS;
+        constexpr int PER = 64 / LPS;
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+          R v = 0;
+#pragma unroll
+          for (int i = 0; i < PER; ++i) v += rvl[((wave * SUB + j) * CW + c) * 64 + pp * PER + i];
+#pragma unroll
+          for (int o = 1; o < LPS; o <<= 1) v += __shfl_xor(v, o, 64);
+          if (pp == 0 && sub + j < cnt)
+            unsafeAtomicAdd(&d.urev[(long long)(qo + wb + sub + j) * w + c], sgn * double(v));
+        }
+        wave_sync_lds();
+      }
+#pragma unroll
+      for (int a = 0; a < TPL * CW; ++a) acc[a] += sgn * double(part[a]);
+      __syncthreads();
+    }
+  }
+  // forward sums: combine the 4 waves, one target slot at a time
+#pragma unroll
+  for (int k = 0; k < TPL; ++k) {
+#pragma unroll
+    for (int c = 0; c < CW; ++c) red[(wave * kTCH + lane) * CW + c] = acc[k * CW + c];
+    __syncthreads();
+    const int t = tp + lane + 64 * k;
+    if (wave == 0 && t < tend) {
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        const double v = red[(0 * kTCH + lane) * CW + c] + red[(1 * kTCH + lane) * CW + c] + red[(2 * kTCH + lane) * CW + c] +
+                         red[(3 * kTCH + lane) * CW + c];
+        atomic_acc(&uo[(long long)(gout + t) * w + c], R(v));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename R, int OP, bool PREC, int UNR>
+__global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
+  using S = Stage<OP, 1>;
+  constexpr int NV = S::nv;
+  constexpr int CW = S::cw;
+  constexpr int TPL = SymTPL<R>::n;
+  constexpr int SUB = SymSub<R, CW>::n;
+  static_assert(OpT<OP>::geom == 0, "symmetric evaluation needs a single-layer kernel");
+  __shared__ R tile[kWG * NV];
+  __shared__ R rvl[4 * SUB * CW * 64];  // reverse partials of one block of sources, per wave
+  __shared__ double red[4 * kTCH * CW];
+
+  const int4 it = d.items[blockIdx.x];
+  const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
+  const int ng = d.g_n[g];
   double ctx = 0, cty = 0, ctz = 0;
   if (PREC) {
     const int gb = d.g_box[g];
@@ -422,129 +558,20 @@ __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
     cty = d.centers[3 * gb + 1];
     ctz = d.centers[3 * gb + 2];
   }
-  const R kappa = R(d.kappa);
   const int tend = min(ng, t0 + kSymTargets);
-
-  for (int tp = t0; tp < tend; tp += 64 * TPL) {  // wave-uniform passes over the item's targets
-    // targets tp + lane + 64 k; padding targets repeat the first one with zero charge
-    R qt[TPL * CW], xt[TPL], yt[TPL], zt[TPL];
-#pragma unroll
-    for (int k = 0; k < TPL; ++k) {
-      const int t = tp + lane + 64 * k;
-      const int sl = gpt + (t < tend ? t : t0);
-      xt[k] = px[sl];
-      yt[k] = py[sl];
-      zt[k] = (S::dim == 3) ? pz[sl] : R(0);
-#pragma unroll
-      for (int c = 0; c < CW; ++c) qt[k * CW + c] = (t < tend) ? qv[(long long)(gout + t) * w + c] : R(0);
-    }
-    double acc[TPL * CW];
-#pragma unroll
-    for (int a = 0; a < TPL * CW; ++a) acc[a] = 0.0;
-
-    for (int s = sb; s < se; ++s) {
-      const int ps = d.e_pt[s], ns = d.e_n[s], qo = d.e_q[s], fl = d.e_flags[s];
-      const double sgn = (fl & SKF_SRC_NEG) ? -1.0 : 1.0;
-      // precision form: sources are shifted into the target box's frame (minus the difference of
-      // the centres), one shift per source for all of a lane's targets
-      R ox = 0, oy = 0, oz = 0;
-      if (PREC) {
-        const int eb = d.e_box[s];
-        ox = R(ctx - d.centers[3 * eb + 0]);
-        oy = R(cty - d.centers[3 * eb + 1]);
-        oz = R(ctz - d.centers[3 * eb + 2]);
-      }
-      for (int base = 0; base < ns; base += kWG) {
-        {
-          const int j = base + tid;
-          R* row = &tile[tid * NV];
-          if (j < ns) {
-            const int sl = ps + j;
-            row[0] = px[sl];
-            row[1] = py[sl];
-            if (S::dim == 3) row[2] = pz[sl];
-            const R* qs = qv + (long long)(qo + j) * w;
-#pragma unroll
-            for (int c = 0; c < CW; ++c) row[S::dim + c] = qs[c];
-          } else {
-#pragma unroll
-            for (int c = 0; c < NV; ++c) row[c] = R(0);
-          }
-        }
-        __syncthreads();
-        const int wb = base + wave * 64;  // wave-uniform
-        const int cnt = __builtin_amdgcn_readfirstlane(max(0, min(64, ns - wb)));
-        const R* tw = &tile[wave * 64 * NV];
-        R part[TPL * CW];
-#pragma unroll
-        for (int a = 0; a < TPL * CW; ++a) part[a] = 0;
-        for (int sub = 0; sub < cnt; sub += SUB) {
-          // rows past the segment end are zero (no charge): blocks are evaluated whole
-#pragma unroll UNR
-          for (int jj = 0; jj < SUB; ++jj) {
-            const R* row = tw + (sub + jj) * NV;
-            const R xs = PREC ? row[0] - ox : row[0];
-            const R ys = PREC ? row[1] - oy : row[1];
-            const R zs = (S::dim == 3) ? (PREC ? row[2] - oz : row[2]) : R(0);
-            R r0 = 0, r1 = 0;
-#pragma unroll
-            for (int k = 0; k < TPL; ++k) {
-              const R dx = xt[k] - xs;
-              const R dy = yt[k] - ys;
-              const R dz = (S::dim == 3) ? zt[k] - zs : R(0);
-              const Cplx<R> kv = kernel_value<R, OP>(dx, dy, dz, R(0), kappa);
-              if (CW == 1) {
-                part[k] += kv.re * row[S::dim];
-                r0 += kv.re * qt[k];
-              } else {
-                const R sr = row[S::dim], si = row[S::dim + 1];
-                part[2 * k] += kv.re * sr - kv.im * si;
-                part[2 * k + 1] += kv.re * si + kv.im * sr;
-                const R tr = qt[2 * k], ti = qt[2 * k + 1];
-                r0 += kv.re * tr - kv.im * ti;
-                r1 += kv.re * ti + kv.im * tr;
-              }
-            }
-            rvl[wave][jj][0][lane] = r0;
-            if (CW == 2) rvl[wave][jj][CW - 1][lane] = r1;
-          }
-          wave_sync_lds();
-          // column sums: lanes j * LPS .. j * LPS + LPS - 1 hold source j's partial over 64 / LPS lanes
-          const int j = lane / LPS, pp = lane % LPS;
-          constexpr int PER = 64 / LPS;
-#pragma unroll
-          for (int c = 0; c < CW; ++c) {
-            R v = 0;
-#pragma unroll
-            for (int i = 0; i < PER; ++i) v += rvl[wave][j][c][pp * PER + i];
-#pragma unroll
-            for (int o = 1; o < LPS; o <<= 1) v += __shfl_xor(v, o, 64);
-            if (pp == 0 && sub + j < cnt)
-              unsafeAtomicAdd(&d.urev[(long long)(qo + wb + sub + j) * w + c], sgn * double(v));
-          }
-          wave_sync_lds();
-        }
-#pragma unroll
-        for (int a = 0; a < TPL * CW; ++a) acc[a] += sgn * double(part[a]);
-        __syncthreads();
-      }
-    }
-    // forward sums: combine the 4 waves, one target slot at a time
-#pragma unroll
-    for (int k = 0; k < TPL; ++k) {
-#pragma unroll
-      for (int c = 0; c < CW; ++c) red[wave][lane][c] = acc[k * CW + c];
-      __syncthreads();
-      const int t = tp + lane + 64 * k;
-      if (wave == 0 && t < tend) {
-#pragma unroll
-        for (int c = 0; c < CW; ++c) {
-          const double v = red[0][lane][c] + red[1][lane][c] + red[2][lane][c] + red[3][lane][c];
-          atomic_acc(&uo[(long long)(gout + t) * w + c], R(v));
-        }
-      }
-      __syncthreads();
-    }
+  // wave-uniform passes; the last one takes as many targets per lane as it needs (no idle slots
+  // beyond the next multiple of 64)
+  for (int tp = t0; tp < tend;) {
+    const int tpl = min(TPL, (tend - tp + 63) / 64);
+    if (tpl >= 4)
+      sym_pass<R, OP, PREC, UNR, 4>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+    else if (tpl == 3)
+      sym_pass<R, OP, PREC, UNR, 3>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+    else if (tpl == 2)
+      sym_pass<R, OP, PREC, UNR, 2>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+    else
+      sym_pass<R, OP, PREC, UNR, 1>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+    tp += 64 * tpl;
   }
 }
 
@@ -598,7 +625,6 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
   const int c0 = blockIdx.y * kMB;
   const int ncv = min(kMB, d.nch - c0);
-  const int nbv = (ncv + 15) / 16;  // valid 16-column blocks (uniform)
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ng = d.g_n[g];
@@ -637,55 +663,47 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   int tot = 0;
   for (int s = sb; s < se; ++s) tot += d.e_n[s];
   int cs = sb, cj = 0;  // stream cursor of the tile start (uniform)
-  for (int base = 0; base < tot; base += 64) {
-    const int nsrc = min(64, tot - base);
-    {
-      const int sr = tid >> 2, cq = (tid & 3) * 16;
-      R* dst = &qt[sr * kQS + cq];
-      if (sr < nsrc) {
-        int s = cs, j = cj + sr;
-        while (j >= d.e_n[s]) {
-          j -= d.e_n[s];
-          ++s;
-        }
-        const int fl = d.e_flags[s];
-        const R sgn = (fl & SKF_SRC_NEG) ? R(-1) : R(1);
-        const R* src = qv + (long long)(d.e_q[s] + j) * d.width + c0 + cq;
-#pragma unroll
-        for (int c = 0; c < 16; ++c) dst[c] = (cq + c < ncv) ? sgn * src[c] : R(0);
-        if ((tid & 3) == 0) {
-          const int sl = d.e_pt[s] + j;
-          R* row = &pt[sr * 8];
-          R ox = 0, oy = 0, oz = 0;
-          if (PREC) {
-            const int eb = d.e_box[s];
-            ox = R(ctx - d.centers[3 * eb + 0]);
-            oy = R(cty - d.centers[3 * eb + 1]);
-            oz = R(ctz - d.centers[3 * eb + 2]);
-          }
-          row[0] = px[sl] - ox;
-          row[1] = py[sl] - oy;
-          row[2] = (DIM == 3) ? pz[sl] - oz : R(0);
-          if (NG) {
-            row[3] = static_cast<const R*>(d.nx)[sl];
-            row[4] = static_cast<const R*>(d.ny)[sl];
-            row[5] = static_cast<const R*>(d.nz)[sl];
-          }
-          pself[sr] = (fl & SKF_SRC_SELF) ? j : -1;
-        }
-      } else {
-#pragma unroll
-        for (int c = 0; c < 16; ++c) dst[c] = R(0);
-        if ((tid & 3) == 0) {
-          // far away, zero charge: contributes exactly zero
-          R* row = &pt[sr * 8];
-          row[0] = R(1e18);
-          row[1] = R(1e18);
-          row[2] = R(1e18);
-          row[3] = row[4] = row[5] = R(0);
-          pself[sr] = -1;
-        }
+  // staging of one tile: global loads into registers, then LDS stores (loading the next tile
+  // during the matrix work was measured slower: the extra registers cost a wave per SIMD)
+  const int sr = tid >> 2, cq = (tid & 3) * 16;
+  R rq[16], rp[6];
+  int rself = -1;
+  auto stage_load = [&](int nsrc) {
+    if (sr < nsrc) {
+      int s = cs, j = cj + sr;
+      while (j >= d.e_n[s]) {
+        j -= d.e_n[s];
+        ++s;
       }
+      const int fl = d.e_flags[s];
+      const R sgn = (fl & SKF_SRC_NEG) ? R(-1) : R(1);
+      const R* src = qv + (long long)(d.e_q[s] + j) * d.width + c0 + cq;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) rq[c] = (cq + c < ncv) ? sgn * src[c] : R(0);
+      if ((tid & 3) == 0) {
+        const int sl = d.e_pt[s] + j;
+        R ox = 0, oy = 0, oz = 0;
+        if (PREC) {
+          const int eb = d.e_box[s];
+          ox = R(ctx - d.centers[3 * eb + 0]);
+          oy = R(cty - d.centers[3 * eb + 1]);
+          oz = R(ctz - d.centers[3 * eb + 2]);
+        }
+        rp[0] = px[sl] - ox;
+        rp[1] = py[sl] - oy;
+        rp[2] = (DIM == 3) ? pz[sl] - oz : R(0);
+        rp[3] = NG ? static_cast<const R*>(d.nx)[sl] : R(0);
+        rp[4] = NG ? static_cast<const R*>(d.ny)[sl] : R(0);
+        rp[5] = NG ? static_cast<const R*>(d.nz)[sl] : R(0);
+        rself = (fl & SKF_SRC_SELF) ? j : -1;
+      }
+    } else {
+      // far away, zero charge: contributes exactly zero
+#pragma unroll
+      for (int c = 0; c < 16; ++c) rq[c] = R(0);
+      rp[0] = rp[1] = rp[2] = R(1e18);
+      rp[3] = rp[4] = rp[5] = R(0);
+      rself = -1;
     }
     // advance the cursor by the tile (uniform)
     cj += nsrc;
@@ -693,7 +711,26 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
       cj -= d.e_n[cs];
       ++cs;
     }
+  };
+  auto stage_store = [&]() {
+    R* dst = &qt[sr * kQS + cq];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) dst[c] = rq[c];
+    if ((tid & 3) == 0) {
+      R* row = &pt[sr * 8];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) row[c] = rp[c];
+      pself[sr] = rself;
+    }
+  };
+  if (tot > 0) {
+    stage_load(min(64, tot));
+    stage_store();
+  }
+  for (int base = 0; base < tot; base += 64) {
+    const int nsrc = min(64, tot - base);
     __syncthreads();
+    const bool more = base + 64 < tot;
     if (wave_busy) {
       V D[4];
 #pragma unroll
@@ -709,9 +746,10 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
         R a = kernel_value<R, OP>(dx, dy, dz, rn, R(0)).re;
         if (pself[sj] == tloc) a = R(0);
         const R* qrow = &qt[sj * kQS + (lane & 15)];
+        // all four column blocks: channels past the last are zero in the staged tile, so the
+        // matrix ops stay branch-free and the accumulators stay in AGPRs
 #pragma unroll
-        for (int nb = 0; nb < 4; ++nb)
-          if (nb < nbv) D[nb] = MF::mac(a, qrow[nb * 16], D[nb]);
+        for (int nb = 0; nb < 4; ++nb) D[nb] = MF::mac(a, qrow[nb * 16], D[nb]);
       }
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb)
@@ -719,6 +757,10 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
         for (int r = 0; r < 4; ++r) acc[nb][r] += double(D[nb][r]);
     }
     __syncthreads();
+    if (more) {
+      stage_load(min(64, tot - base - 64));
+      stage_store();
+    }
   }
   R* uo = static_cast<R*>(d.u);
 #pragma unroll

@@ -249,7 +249,7 @@ class GPUSkeletonizer:
 
     # ---------------------------------------------------------------- sketched
     def sketched(self, cands, centers, length, nproxy, k_pred, oversample=1.25, extra=8, cut=0.25, accept=4.0,
-                 check_rows=16, direct=0.9):
+                 check_rows=16, direct=0.9, keep_all=0.98):
         """Learned-rank IDs from Gaussian sketches; None where the box must fall back.
 
         Box b is sketched with ``l_b = min(m, n_b, ceil(oversample k_b) + extra)`` rows; the
@@ -259,7 +259,10 @@ class GPUSkeletonizer:
         ``check_rows``-row sketch is within ``accept * tol`` (``accept_sketch``, computed on
         the GPU by ``skf_id_residual``). Boxes predicted to be nearly incompressible
         (``k_b >= direct * min(m, n_b)``: a sketch that size distorts too much to be
-        accepted) are not sketched; they return ``"direct"`` and take the adaptive path.
+        accepted) are not sketched; they return ``"direct"`` and take the adaptive path. Boxes
+        predicted full rank (``k_b >= keep_all * n_b`` with ``n_b <= m``) keep every candidate
+        as skeleton (rank ``n_b``, empty ``T``: exact, no factorization) and return it tagged
+        ``"kept"``; ``keep_all=None`` disables this.
         """
         t0 = perf_counter()
         self._t = perf_counter()
@@ -273,7 +276,11 @@ class GPUSkeletonizer:
         kp = np.asarray(k_pred, dtype=np.float64)
         ok = []
         for i in range(nb):
-            if kp[i] >= direct * min(m, int(ncols_all[i])):
+            n_i = int(ncols_all[i])
+            if keep_all is not None and n_i <= m and kp[i] >= keep_all * n_i:
+                results[i] = ("kept", (n_i, np.arange(n_i, dtype=np.int64),
+                                       np.zeros((n_i, 0), dtype=np.complex128 if self.cplx else np.float64)))
+            elif kp[i] >= direct * min(m, n_i):
                 results[i] = "direct"
             elif self._lds_ok(m, int(ncols_all[i])):
                 ok.append(i)

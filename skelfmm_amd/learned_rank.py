@@ -272,9 +272,11 @@ def harvest_suite(n_builds=48, seed=0, max_points=60000, verbose=False, suite="m
     large 3D Gaussian mixtures (8-32 components, sigma 0.01-0.05), Helmholtz or Laplace,
     tol 1e-5..1e-7, leaf 128 or 256 -- deep adaptive trees.
 
-    The committed set ``data/rank_pairs.npz`` is ``harvest --builds 72 --seed 0
-    --max-points 80000``, then ``--builds 32 --seed 1 --max-points 80000 --append``,
-    then ``--suite clustered --builds 16 --seed 2 --max-points 600000 --append``.
+    The committed set ``data/rank_pairs.npz`` (383,128 pairs from 119 builds) is
+    ``harvest --builds 72 --seed 0 --max-points 80000``, then ``--builds 32 --seed 1
+    --max-points 80000 --append``, then ``--suite clustered --builds 16 --seed 2
+    --max-points 600000 --append`` (one of those 16 builds is skipped: clipped points pile up
+    on a cube corner beyond the leaf size).
     """
     O = operators_mod
     if suite == "clustered":

@@ -320,7 +320,8 @@ def test_fp32_double_layer_surface_conditioning():
     """Double layer on the unit sphere with normals = positions: rn = (x - y).n = -r^2/2 is
     ill-conditioned in the coordinates (rounding only the points to FP32 costs ~4e-3 in an exact
     sum). The box-centred FP32 form must do at least as well as the oracle's restatement of the
-    reference's single-precision form on the same build, with FP32 matrix-core and VALU paths equal."""
+    reference's single-precision form on the same build, and the FP32 matrix-core and VALU paths
+    (which round the box-centre shift at different places) must differ by far less than that error."""
     from skelfmm_amd.batched import SkelFMMBatched
 
     X = sphere_points(5000, 4)
@@ -337,7 +338,7 @@ def test_fp32_double_layer_surface_conditioning():
     e_gpu = rel_l2(y_mfma, y64)
     print(f"sphere DL FP32: oracle single-precision form {e_ref:.2e}, GPU precision form {e_gpu:.2e}", flush=True)
     assert e_gpu <= e_ref
-    assert rel_l2(y_mfma, y_valu) < 1e-5
+    assert rel_l2(y_mfma, y_valu) < 0.02 * e_gpu
 
 
 def test_autotune_picks_setting_and_keeps_parity(tmp_path, monkeypatch):
