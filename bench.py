@@ -55,7 +55,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # rocprofv3 --pmc passes summarised in profiles/apply_laplace3d_uniform1M_hbm_traffic.md
 # (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, scale factors from a 512 MiB copy of the same run)
 P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400, False): 2549.1e6 + 14.2e6,
-                          (1_000_000, 1e-6, 400, True): 536.9e6 + 639.8e6}
+                          (1_000_000, 1e-6, 400, True): 523.3e6 + 672.1e6}
 
 
 def log(msg):
@@ -508,6 +508,7 @@ def main():
     q32 = q64.to(torch.float32)
 
     def timed(devapply, x):
+        """(graph-replay s per apply, eager s per apply, result): K steps each after W warmups."""
         y = torch.empty_like(x)
         for _ in range(args.warmup):
             devapply.apply_device(x, out=y)
@@ -516,11 +517,30 @@ def main():
         for _ in range(args.steps):
             devapply.apply_device(x, out=y)
         torch.cuda.synchronize()
-        return (time.perf_counter() - ts) / args.steps, y
+        t_eager = (time.perf_counter() - ts) / args.steps
+        g = devapply.capture(x, y)
+        for _ in range(args.warmup):
+            g.replay()
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - ts) / args.steps, t_eager, y
 
-    t64, y64 = timed(d64, q64)
-    t32, y32 = timed(d32, q32)
-    log(f"apply FP64 {t64 * 1e3:.2f} ms, FP32 {t32 * 1e3:.2f} ms, ratio {t64 / t32:.2f}")
+    t64, e64_t, y64 = timed(d64, q64)
+    t32, e32_t, y32 = timed(d32, q32)
+    log(f"apply (graph replay) FP64 {t64 * 1e3:.2f} ms, FP32 {t32 * 1e3:.2f} ms, ratio {t64 / t32:.2f}; "
+        f"eager FP64 {e64_t * 1e3:.2f} ms, FP32 {e32_t * 1e3:.2f} ms")
+
+    # the drop-in call: host numpy charges in, host numpy potentials out (PCIe copies included)
+    qh32 = q.astype(np.float32)
+    d32.matvec(torch.as_tensor(qh32)).numpy()
+    ts = time.perf_counter()
+    for _ in range(3):
+        d32.matvec(torch.as_tensor(qh32)).numpy()
+    t_host32 = (time.perf_counter() - ts) / 3
+    log(f"host-buffer apply FP32 (numpy in/out): {t_host32 * 1e3:.2f} ms")
 
     # near-field launch alone (the plan's own arguments), HIP events on the stream it runs on
     def p2p_time(devapply, width=1, reps=10):
@@ -589,7 +609,11 @@ def main():
                    "interactions_per_apply": int(d64.interactions),
                    "autotune": None if tuned is None else {k: {"max_sources": v["max_sources"]}
                                                            for k, v in tuned.items()}},
-        "fp64": {"value": round(N / t64 / 1e6, 3), "ms_per_step": round(t64 * 1e3, 4), "rel_l2_err": e64},
+        "fp64": {"value": round(N / t64 / 1e6, 3), "ms_per_step": round(t64 * 1e3, 4), "rel_l2_err": e64,
+                 "eager_ms_per_step": round(e64_t * 1e3, 4)},
+        "timing": "HIP graph replays of the whole apply (charges and result resident in HBM)",
+        "eager_ms_per_step": round(e32_t * 1e3, 4),
+        "host_buffer_apply_ms": round(t_host32 * 1e3, 3),
         "fp32_rel_l2_err": e32,
         "fp32_over_fp64_speedup": round(t64 / t32, 3),
         "build_s": round(build_s, 2),

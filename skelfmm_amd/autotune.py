@@ -26,7 +26,7 @@ import os
 
 import torch
 
-SPLITS = (4096, 8192, 16384, 32768, 65536)
+SPLITS = (1024, 2048, 4096, 8192, 16384, 32768, 65536)
 
 
 def cache_path():

@@ -51,6 +51,44 @@ class SkelFMMBatched(torch.nn.Module):
         self._graphs = {}
 
     # ------------------------------------------------------------------ build
+    @classmethod
+    def from_layout(cls, L, operator, kappa=0.0, use_single_precision=False, device=None, tol=None, nlevels=None):
+        """Device apply from a box layout alone, e.g. one read back by ``serialize.load_layout``."""
+        self = cls.__new__(cls)
+        torch.nn.Module.__init__(self)
+        self.operator = operators_mod._require_operator(operator)
+        if self.operator.kernel_id < 0:
+            raise NotImplementedError("the HIP apply supports the built-in operators of skelfmm_amd.operators only")
+        self.kappa = float(kappa)
+        self.ndim = int(L.ndim)
+        self.N = int(L.N)
+        self.nboxes = int(L.nboxes)
+        self.nlevels = nlevels
+        self.tree = None
+        self.tol = tol
+        self.root_level = int(L.root_level)
+        self.kernel_backend = "hip"
+        self._plans = {}
+        self._graphs = {}
+        return self.upload_layout(L, use_single_precision, device)
+
+    def save(self, path, *, points=None, geometry=None, leaf_size=-1, extra=None, extra_arrays=None):
+        """Write this apply's layout and operator to a versioned ``.npz`` (``serialize.save_layout``).
+
+        ``points`` (caller's order) default to the layout's own coordinates."""
+        from .serialize import save_layout
+
+        L = self.layout
+        if points is None:
+            points = np.empty((L.N, L.ndim))
+            points[L.leaf_user] = L.points[:, L.leaf_slot].T
+        if geometry is None and L.normals is not None:
+            geometry = np.empty((L.N, 3))
+            geometry[L.leaf_user] = L.normals[:, L.leaf_slot].T
+        save_layout(path, L, operator_name=self.operator.name, kappa=self.kappa,
+                    tol=-1.0 if getattr(self, "tol", None) is None else self.tol, leaf_size=leaf_size,
+                    points=points, geometry=geometry, extra=dict(extra or {}, nlevels=self.nlevels),
+                    extra_arrays=extra_arrays)
     def build_from_copy(self, skel_fmm, use_single_precision=False, device=None, max_sources=16384):
         self.tol = skel_fmm.tol
         self.root_level = skel_fmm.root_level

@@ -8,13 +8,15 @@ Two paths for the boxes of one tree level, both in FP64 (complex128 for Helmholt
   reference's adaptive ID (``skelfmm/util.py:13-51``) with early stopping instead
   of a full factorization.
 * ``sketched`` (the learned-rank build): given a predicted rank ``k_b`` per box, a
-  Gaussian sketch ``Y_b = Omega A_b`` with ``l_b = ceil(1.5 k_b) + 16`` rows (one
+  Gaussian sketch ``Y_b = Omega A_b`` with ``l_b = ceil(1.25 k_b) + 8`` rows (one
   batched GEMM) replaces ``A_b`` in the pivoted QR, which then runs on ``l_b`` rows
-  instead of ``m``, with the cutoff tightened to ``tol / 4`` (a Gaussian sketch blurs
+  instead of ``m``, with the cutoff tightened to ``tol / 2`` (a Gaussian sketch blurs
   it). A box is accepted when the QR of its sketch reaches the cutoff within ``l_b``
   steps and an independent 16-row sketch of the residual ``A[:, red] - A[:, skel] T``
-  stays within ``4 tol`` of the largest sketched column norm (the proxy-residual
+  stays within ``6 tol`` of the largest sketched column norm (the proxy-residual
   check, ``skf_id_residual``); every other box falls back to the adaptive path.
+  Boxes predicted nearly incompressible take the adaptive path directly, boxes
+  predicted full rank keep all their points without a factorization.
 
 Results are returned in the CPU build's convention: ``(rank, idx, proj)`` with
 ``idx`` the column pivots and ``proj = T``.
@@ -47,7 +49,7 @@ def _unit_proxy(operator, ndim, nproxy):
     return out
 
 
-def accept_sketch(Z, result, lrows, tol, cap=None, accept=4.0):
+def accept_sketch(Z, result, lrows, tol, cap=None, accept=6.0):
     """Acceptance test of a sketched ID ``result = (rank, idx, T)`` from ``lrows`` sketch rows.
 
     Rejected when the sketch's QR ran out of rows before the rank cutoff
@@ -248,8 +250,8 @@ class GPUSkeletonizer:
         return results
 
     # ---------------------------------------------------------------- sketched
-    def sketched(self, cands, centers, length, nproxy, k_pred, oversample=1.25, extra=8, cut=0.25, accept=4.0,
-                 check_rows=16, direct=0.9, keep_all=0.98):
+    def sketched(self, cands, centers, length, nproxy, k_pred, oversample=1.25, extra=8, cut=0.5, accept=6.0,
+                 check_rows=16, direct=0.9, keep_all=0.95):
         """Learned-rank IDs from Gaussian sketches; None where the box must fall back.
 
         Box b is sketched with ``l_b = min(m, n_b, ceil(oversample k_b) + extra)`` rows; the

@@ -63,6 +63,25 @@ def _level_tables(fmm, boxes, off, toff):
             toff[boxes].astype(np.int64))
 
 
+def _ranges(counts):
+    """Concatenation of arange(c) for c in counts."""
+    counts = np.asarray(counts, dtype=np.int64)
+    tot = int(counts.sum())
+    if tot == 0:
+        return np.zeros(0, np.int64)
+    starts = np.repeat(np.cumsum(counts) - counts, counts)
+    return np.arange(tot, dtype=np.int64) - starts
+
+
+def _box_items(lengths, step):
+    """Items {box index, first row/column} covering lengths[i] in steps of ``step``."""
+    lengths = np.asarray(lengths, dtype=np.int64)
+    cnt = (lengths + step - 1) // step
+    idx = np.repeat(np.arange(lengths.shape[0], dtype=np.int64), cnt)
+    first = _ranges(cnt) * step
+    return np.stack([idx, first], axis=1).astype(np.int32).reshape(-1, 2)
+
+
 def build_layout(fmm, max_sources=16384):
     """Flatten a ``RecursiveSkeletonization`` build into device-ready arrays."""
     tree = fmm.tree
@@ -82,11 +101,9 @@ def build_layout(fmm, max_sources=16384):
     L.nslots = nslots
     L.off = off
 
-    slot_pt = np.zeros(nslots, dtype=np.int64)
-    slot_box = np.zeros(nslots, dtype=np.int64)
-    for b in np.nonzero(has)[0]:
-        slot_pt[off[b]:off[b] + bs[b]] = fmm.cand[b]
-        slot_box[off[b]:off[b] + bs[b]] = b
+    hb = np.nonzero(has)[0]
+    slot_box = np.repeat(hb, bs[hb]).astype(np.int64)
+    slot_pt = (np.concatenate([fmm.cand[b] for b in hb]) if hb.size else np.zeros(0)).astype(np.int64)
     L.slot_point = slot_pt
     L.slot_box = slot_box
     L.points = np.ascontiguousarray(tree.XX[slot_pt].T)  # (d, nslots)
@@ -137,32 +154,36 @@ def build_layout(fmm, max_sources=16384):
         sboxes = fmm.skel_boxes_level(lev)
         sboxes = sboxes[rk[sboxes] > 0]
         tabs = _level_tables(fmm, sboxes, off, toff)
-        items = [(i, r0) for i, b in enumerate(sboxes) for r0 in range(0, int(rk[b]), ROWS_PER_ITEM)]
-        items = np.array(items, dtype=np.int32).reshape(-1, 2)
+        items = _box_items(rk[sboxes], ROWS_PER_ITEM)
         up.append(LevelStage(lev, *tabs, gather_dst, gather_src, items))
     L.up = up
 
     # downward stages: shallowest skeletonized level first
-    pos_in_parent = {}
-    for P in range(nb):
-        cs = fmm.child_src[P]
-        if cs is not None and cs.shape[0] > 0 and bs[P] > 0:
-            for k, (c, j) in enumerate(cs):
-                pos_in_parent[(int(c), int(j))] = off[P] + k
+    # slot of child skeleton point (c, j) inside its parent: parents hold their candidates in
+    # skeleton-first order, child_src[P][k] = (c, j) for slot off[P] + k; looked up by sorted key
+    parents = [P for P in range(nb) if fmm.child_src[P] is not None and fmm.child_src[P].shape[0] > 0 and bs[P] > 0]
+    maxj = int(rk.max()) + 1 if rk.size else 1
+    if parents:
+        cs_all = np.concatenate([fmm.child_src[P] for P in parents]).astype(np.int64)
+        pos_all = np.concatenate([off[P] + np.arange(fmm.child_src[P].shape[0]) for P in parents])
+        keys = cs_all[:, 0] * maxj + cs_all[:, 1]
+        korder = np.argsort(keys, kind="stable")
+        keys_sorted, pos_sorted = keys[korder], pos_all[korder]
+    else:
+        keys_sorted = pos_sorted = np.zeros(0, np.int64)
     down = []
     for lev in range(fmm.root_level + 1, fmm.nlevels):
         sboxes = fmm.skel_boxes_level(lev)
         sboxes = sboxes[rk[sboxes] > 0]
-        gd, gs = [], []
-        for B in sboxes:
-            r = int(rk[B])
-            gd.append(np.arange(off[B], off[B] + r))
-            gs.append(np.array([pos_in_parent[(int(B), j)] for j in range(r)], dtype=np.int64))
-        gather_dst = np.concatenate(gd).astype(np.int32) if gd else np.zeros(0, np.int32)
-        gather_src = np.concatenate(gs).astype(np.int32) if gs else np.zeros(0, np.int32)
+        r = rk[sboxes].astype(np.int64)
+        within = _ranges(r)
+        gather_dst = (np.repeat(off[sboxes], r) + within).astype(np.int32)
+        q = np.repeat(sboxes.astype(np.int64), r) * maxj + within
+        at = np.searchsorted(keys_sorted, q)
+        assert np.all(at < keys_sorted.shape[0]) and np.all(keys_sorted[np.minimum(at, max(keys_sorted.shape[0] - 1, 0))] == q)
+        gather_src = pos_sorted[at].astype(np.int32)
         tabs = _level_tables(fmm, sboxes, off, toff)
-        items = [(i, c0) for i, b in enumerate(sboxes) for c0 in range(0, int(bs[b] - rk[b]), COLS_PER_ITEM)]
-        items = np.array(items, dtype=np.int32).reshape(-1, 2)
+        items = _box_items(bs[sboxes] - rk[sboxes], COLS_PER_ITEM)
         down.append(LevelStage(lev, *tabs, gather_dst, gather_src, items))
     L.down = down
 
@@ -302,31 +323,38 @@ def make_items(g_n, g_e0, g_e1, e_n, max_sources, targets_per_item=TARGETS_PER_I
     """Near-field work items {group, first target, first entry, end entry}, heaviest first.
 
     Each item covers up to ``targets_per_item`` targets of one group and a run of that
-    group's source entries holding at most ``max_sources`` points (at least one entry).
+    group's consecutive source entries: the entries whose first source point falls in the
+    same window of ``max_sources`` points of the group's source list (a run holds at most
+    ``max_sources`` plus one entry's points, and at least one entry).
     """
-    items = []
-    costs = []
-    cums = np.concatenate(([0], np.cumsum(np.asarray(e_n, dtype=np.int64))))
-    for g in range(len(g_n)):
-        s0, s1 = int(g_e0[g]), int(g_e1[g])
-        ranges = []
-        a = s0
-        while a < s1:
-            b = a + 1
-            while b < s1 and cums[b + 1] - cums[a] <= max_sources:
-                b += 1
-            ranges.append((a, b))
-            a = b
-        ng = int(g_n[g])
-        for t0 in range(0, ng, targets_per_item):
-            nt = min(targets_per_item, ng - t0)
-            for a, b in ranges:
-                items.append((g, t0, a, b))
-                costs.append(nt * int(cums[b] - cums[a]))
-    items = np.array(items, dtype=np.int32).reshape(-1, 4)
-    costs = np.array(costs, dtype=np.int64)
+    g_n = np.asarray(g_n, dtype=np.int64)
+    g_e0 = np.asarray(g_e0, dtype=np.int64)
+    g_e1 = np.asarray(g_e1, dtype=np.int64)
+    e_n = np.asarray(e_n, dtype=np.int64)
+    ne = g_e1 - g_e0
+    if g_n.shape[0] == 0 or ne.sum() == 0:
+        return np.zeros((0, 4), dtype=np.int32), np.zeros(0, dtype=np.int64)
+    cums = np.concatenate(([0], np.cumsum(e_n)))
+    gid = np.repeat(np.arange(g_n.shape[0], dtype=np.int64), ne)
+    ent = np.repeat(g_e0, ne) + _ranges(ne)
+    loc = cums[ent] - cums[g_e0[gid]]
+    bucket = loc // int(max_sources)
+    new_run = np.ones(ent.shape[0], dtype=bool)
+    new_run[1:] = (gid[1:] != gid[:-1]) | (bucket[1:] != bucket[:-1])
+    ra = ent[new_run]
+    rg = gid[new_run]
+    rb = np.concatenate((ra[1:], [0]))
+    last_of_group = np.concatenate((rg[1:] != rg[:-1], [True]))
+    rb[last_of_group] = g_e1[rg[last_of_group]]
+    rsrc = cums[rb] - cums[ra]
+    nblk = (g_n[rg] + targets_per_item - 1) // targets_per_item
+    item_run = np.repeat(np.arange(ra.shape[0]), nblk)
+    t0 = _ranges(nblk) * targets_per_item
+    ig = rg[item_run]
+    items = np.stack([ig, t0, ra[item_run], rb[item_run]], axis=1).astype(np.int32)
+    costs = np.minimum(targets_per_item, g_n[ig] - t0) * rsrc[item_run]
     order = np.argsort(-costs, kind="stable")
-    return np.ascontiguousarray(items[order]), costs[order]
+    return np.ascontiguousarray(items[order]), costs[order].astype(np.int64)
 
 
 def resplit(L, max_sources):

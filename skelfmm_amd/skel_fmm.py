@@ -8,7 +8,9 @@ and ``benchmark_matvec`` follow the reference. Opt-in extension: ``apply``
 also accepts an ``(N, R)`` array of right-hand sides when ``multi_rhs=True``;
 ``build_device="gpu"`` runs the per-box interpolative decompositions as batched
 HIP kernels, and ``rank_model`` (a ``learned_rank.RankMLP`` or ``"default"``)
-switches them to the learned fixed-rank sketched form with adaptive fallback.
+switches them to the learned fixed-rank sketched form with adaptive fallback;
+``save(path)`` / ``SkelFMM.load(path)`` write and read the built operator as one
+versioned file (``skelfmm_amd.serialize``).
 """
 
 from __future__ import annotations
@@ -185,6 +187,50 @@ class SkelFMM:
         self.device_storage_bytes = device_storage
         self.total_storage_bytes = host_storage + device_storage
         self.nbytes = self.total_storage_bytes
+        self._is_built = True
+        return self
+
+    # --------------------------------------------------------- serialization
+    def save(self, path):
+        """Write the built operator to ``path`` (versioned ``.npz``; ``skelfmm_amd.serialize``)."""
+        self._ensure_built()
+        self.batched.save(path, points=np.asarray(self.points), geometry=self.geometry, leaf_size=self.leaf_size,
+                          extra={"nboxes": int(self.nboxes), "nleaves": int(self.nleaves),
+                                 "rank_max": int(self.rank_max), "build_device": self.build_device,
+                                 "build_time_seconds": float(self.build_time_seconds)})
+
+    @classmethod
+    def load(cls, path, *, use_single_precision=False, multi_rhs=False, quiet=True, verify=True):
+        """A built ``SkelFMM`` from a file written by ``save``: no tree, IDs or lists are formed."""
+        from . import operators as ops_mod
+        from .batched import SkelFMMBatched
+        from .serialize import load_layout
+
+        t0 = perf_counter()
+        rec = load_layout(path, verify=verify)
+        h = rec.header
+        by_name = {op.name: op for op in ops_mod.BUILTIN_OPERATORS}
+        if h["operator"] not in by_name:
+            raise ValueError(f"{path}: unknown operator {h['operator']!r}")
+        self = cls(rec.points, by_name[h["operator"]], geometry=rec.geometry, tol=h["tol"],
+                   leaf_size=h["leaf_size"], use_single_precision=use_single_precision, param=h["kappa"],
+                   quiet=quiet, defer_build=True, multi_rhs=multi_rhs)
+        ex = h.get("extra", {})
+        self.batched = SkelFMMBatched.from_layout(rec.layout, self.operator, kappa=h["kappa"],
+                                                  use_single_precision=use_single_precision, tol=h["tol"],
+                                                  nlevels=ex.get("nlevels"))
+        self.nlevels = ex.get("nlevels") or 0
+        self.nboxes = ex.get("nboxes", int(rec.layout.nboxes))
+        self.nleaves = ex.get("nleaves", 0)
+        self.rank_max = ex.get("rank_max", 0)
+        self.build_device = ex.get("build_device", "cpu")
+        self.device = self.batched.device
+        self.kernel_backend = self.batched.kernel_backend
+        self.device_storage_bytes = int(self.batched.nbytes_proj() + self.batched.nbytes_lists())
+        self.total_storage_bytes = self.device_storage_bytes
+        self.nbytes = self.total_storage_bytes
+        self.setup_time_seconds = perf_counter() - t0
+        self.build_time_seconds = self.setup_time_seconds
         self._is_built = True
         return self
 

@@ -290,14 +290,18 @@ def test_mfma_multi_rhs_precision_form(name, nrhs):
 
 
 @pytest.mark.parametrize("single", [False, True])
-def test_graph_replay_matches_eager(single):
+@pytest.mark.parametrize("nrhs", [1, 2])
+def test_graph_replay_matches_eager(single, nrhs):
+    """Replays of a captured apply equal the eager apply; 1 RHS runs the symmetric near field
+    (zero fill of the reverse-sum buffer, pairs of lists, fold), 2 RHS the one-sided kernel."""
     from skelfmm_amd.batched import SkelFMMBatched
 
     X = np.random.default_rng(3).random((20000, 3))
     fmm = make_build(X, ops.LAPLACE_3D, tol=1e-6, leaf_size=150)
     dev = SkelFMMBatched(fmm.tree, ops.LAPLACE_3D).build_from_copy(fmm, use_single_precision=single)
+    assert nrhs > 1 or dev.use_symmetric
     dt = torch.float32 if single else torch.float64
-    x = torch.randn(X.shape[0], 2, dtype=dt, device=_gpu())
+    x = torch.randn(X.shape[0], nrhs, dtype=dt, device=_gpu()).squeeze(-1)
     out = torch.empty_like(x)
     eager = dev.apply_device(x).clone()
     g = dev.capture(x, out)
