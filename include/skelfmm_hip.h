@@ -44,7 +44,10 @@ enum skf_kernel {
   SKF_LAPLACE_DL_3D = 4,
   SKF_LAPLACE_ADL_3D = 5,
   SKF_HELMHOLTZ_DL_3D = 6,
-  SKF_HELMHOLTZ_ADL_3D = 7
+  SKF_HELMHOLTZ_ADL_3D = 7,
+  /* user-defined real point kernels, near field compiled at run time (skf_jit_compile) */
+  SKF_USER_2D = 16,
+  SKF_USER_3D = 17
 };
 
 enum skf_status {
@@ -135,6 +138,9 @@ typedef struct skf_p2p_args {
   double* urev;
   long long urev_len;
   int sym_unroll;              /* sources per inner step of the symmetric kernel: 1, 2, 4 (0 = 2) */
+  /* kernel = SKF_USER_2D / SKF_USER_3D: run-time compiled near-field kernels (hipFunction_t from
+     skf_jit_compile) for channel blocks of 1, 2, 4 and 8, in this argument's precision */
+  void* user_fn[4];
 } skf_p2p_args;
 
 /* one level of the upward or downward interpolation pass */
@@ -206,6 +212,22 @@ int skf_apply(skf_plan* p, const void* x, void* y, void* stream);
 int skf_p2p_blocks(int kernel, int precision, int width, double kappa, int nblocks, int m, int n, int ndim,
                    const void* xt, const void* xs, const void* nt, const void* ns, const void* q, void* u,
                    int self_interaction, void* stream);
+
+/*
+ * Run-time compiled near-field kernels for user-defined operators (SKF_USER_2D / _3D).
+ * Reference: the KernelOperator plugin (skelfmm/operators.py:17-108) whose lazy kernels the
+ * reference evaluates through PyKeOps formulas (skelfmm/util_batched.py:189-224); here the
+ * operator's formula becomes a kernel_value<> specialization compiled with hiprtc.
+ * source: the translation unit (it includes header_name); names: template instances to load
+ * (e.g. "p2p_kernel<double, 17, 1, false>"). module = NULL compiles only (no device needed)
+ * and reports the code size; otherwise the module is loaded and functions[i] receives the
+ * hipFunction_t of names[i]. skf_jit_log() holds the compiler log of the last call.
+ */
+int skf_jit_compile(const char* source, const char* header_name, const char* header_source,
+                    const char* const* names, int nnames, void** module, void** functions,
+                    long long* code_bytes);
+const char* skf_jit_log(void);
+int skf_jit_unload(void* module);
 
 /* ---------------------------------------------------------------- GPU build
  * Reference: the per-box interpolative decomposition of the build,

@@ -74,7 +74,7 @@ def tune(dev, width=1, reps=3, use_cache=True, splits=SPLITS):
     if key in db:
         apply_setting(dev, db[key])
         return dict(db[key], cached=True)
-    nch = width // 2 if (dev.operator.is_complex or dev.kappa > 0) else width
+    nch = width // 2 if dev.complex_vectors else width
     mfma_opts = [0]
     if dev.precision == 1 and not dev.operator.is_complex and nch >= 16:
         mfma_opts = [0, -1]

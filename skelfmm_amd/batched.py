@@ -133,7 +133,7 @@ class SkelFMMBatched(torch.nn.Module):
         if L.normals is not None:
             self.normals = torch.as_tensor(L.normals, dtype=rd, device=dev).contiguous()
         self.centers = torch.as_tensor(L.centers, dtype=torch.float64, device=dev).contiguous()
-        self.complex_t = bool(np.iscomplexobj(L.T))
+        self.complex_t = bool(L.T.is_complex()) if torch.is_tensor(L.T) else bool(np.iscomplexobj(L.T))
         if self.complex_t:
             Tt = torch.as_tensor(L.T).to(self.complex_dtype)
             self.T = torch.view_as_real(Tt).reshape(-1).to(dev).contiguous()
@@ -244,6 +244,12 @@ class SkelFMMBatched(torch.nn.Module):
         a.nitems = int(nf["items"].shape[0] // 4)
         a.mfma_min_channels = int(self.mfma_min_channels)
         a.sym_unroll = int(self.sym_unroll)
+        uk = getattr(self.operator, "user_kernel", None)
+        if uk is not None:  # user-defined operator: near field compiled at run time
+            from .jit import compile_kernels
+
+            for i, fn in enumerate(compile_kernels(uk, self.precision)):
+                a.user_fn[i] = fn
         if self.use_symmetric and (self.precision == 0 or urev is not None):
             a.one = ctypes.pointer(self._split_structs[0])
             a.sym = ctypes.pointer(self._split_structs[1])
@@ -254,7 +260,7 @@ class SkelFMMBatched(torch.nn.Module):
 
     def _urev(self, width):
         """FP64 buffer for the symmetric kernel's reverse sums (FP32 form, one channel)."""
-        cx = self.operator.is_complex or self.kappa > 0
+        cx = self.complex_vectors
         if not self.use_symmetric or self.precision != 1 or width != (2 if cx else 1):
             return None
         return torch.zeros(2 * self.nslots * width, dtype=torch.float64, device=self.device)
@@ -302,8 +308,15 @@ class SkelFMMBatched(torch.nn.Module):
             pass
 
     # ------------------------------------------------------------------ apply
+    @property
+    def complex_vectors(self):
+        """Charges and potentials are complex: complex kernels, and (as the reference casts them,
+        ``batched_fmm.py:539-564``) every built-in kernel with kappa > 0; a user-defined real
+        kernel keeps real vectors whatever its parameter."""
+        return bool(self.operator.is_complex or (self.kappa > 0 and getattr(self.operator, "user_kernel", None) is None))
+
     def _vector_dtype(self, q):
-        cplx = q.is_complex() or self.kappa > 0 or self.operator.is_complex
+        cplx = q.is_complex() or self.complex_vectors
         return self.complex_dtype if cplx else self.real_dtype
 
     def prepare_input(self, q):

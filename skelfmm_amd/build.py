@@ -266,6 +266,9 @@ class RecursiveSkeletonization:
         # + no model), host IDs (LDS too small)
         self.gpu_stats = {"sketched": 0, "fallback": 0, "direct": 0, "kept": 0, "adaptive": 0, "cpu": 0}
         gpu = None
+        if device == "gpu" and getattr(self.operator, "user_kernel", None) is not None:
+            raise ValueError("user-defined operators are built on the host (device='cpu'); "
+                             "their near field runs on the GPU")
         if device == "gpu":
             from .gpu_build import GPUSkeletonizer
 

@@ -46,316 +46,9 @@ static int set_err(int code, const char* fmt, ...) {
 extern "C" const char* skf_last_error(void) { return g_err; }
 extern "C" int skf_version(void) { return 1; }
 
-// ---------------------------------------------------------- kernel algebra
-// Closed forms of skelfmm/util_batched.py (torch path): distances are clamped
-// from below (r^2 >= 1e-15 for the cdist-based kernels, >= 1e-30 for the
-// Laplace double layers in FP64), self pairs (i == j inside a self segment)
-// contribute zero. The FP32 form uses r^2 + 1e-20 for every kernel (the bias is
-// below FP32 resolution for any r^2 > 1e-12 and keeps coincident points finite).
+#include "p2p_core.h"
 
-template <typename R>
-struct Cplx {
-  R re, im;
-};
-
-template <typename R>
-struct Mth;
-
-template <>
-struct Mth<double> {
-  static constexpr bool fp32_bias = false;
-  static constexpr double clamp_cd = 1e-15;
-  static constexpr double clamp_dl = 1e-30;
-  // 1/sqrt(x) for normal x > 0: hardware estimate (relative error ~5e-8) refined by one Newton
-  // step with its second-order term, y (1 + e/2 + 3e^2/8), e = 1 - x y^2 (measured ~1 ulp).
-  __device__ static inline double rsq(double x) {
-    const double y = __builtin_amdgcn_rsq(x);
-    const double e = fma(-x * y, y, 1.0);
-    return fma(y * e, fma(0.375, e, 0.5), y);
-  }
-  __device__ static inline double lg(double x) { return log(x); }
-  __device__ static inline double half_lg(double x) { return 0.5 * log(x); }
-  __device__ static inline void sc(double x, double* s, double* c) { sincos(x, s, c); }
-  __device__ static inline double bj0(double x) { return j0(x); }
-  __device__ static inline double by0(double x) { return y0(x); }
-};
-
-template <>
-struct Mth<float> {
-  static constexpr bool fp32_bias = true;
-  static constexpr float clamp_cd = 1e-20f;
-  static constexpr float clamp_dl = 1e-20f;
-  __device__ static inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
-  // natural log from the hardware log2 (x >= 1e-20 here: normal, no denormal scaling needed)
-  __device__ static inline float lg(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
-  __device__ static inline float half_lg(float x) { return __builtin_amdgcn_logf(x) * 0.34657359027997265f; }
-  __device__ static inline void sc(float x, float* s, float* c) { sincosf(x, s, c); }
-  __device__ static inline float bj0(float x) { return j0f(x); }
-  __device__ static inline float by0(float x) { return y0f(x); }
-};
-
-// Traits per operator: spatial dimension, complex value, where normals live.
-template <int OP>
-struct OpT;
-template <> struct OpT<SKF_LAPLACE_2D> { static constexpr int dim = 2, cplx = 0, geom = 0; };
-template <> struct OpT<SKF_LAPLACE_3D> { static constexpr int dim = 3, cplx = 0, geom = 0; };
-template <> struct OpT<SKF_HELMHOLTZ_2D> { static constexpr int dim = 2, cplx = 1, geom = 0; };
-template <> struct OpT<SKF_HELMHOLTZ_3D> { static constexpr int dim = 3, cplx = 1, geom = 0; };
-template <> struct OpT<SKF_LAPLACE_DL_3D> { static constexpr int dim = 3, cplx = 0, geom = 1; };
-template <> struct OpT<SKF_LAPLACE_ADL_3D> { static constexpr int dim = 3, cplx = 0, geom = 2; };
-template <> struct OpT<SKF_HELMHOLTZ_DL_3D> { static constexpr int dim = 3, cplx = 1, geom = 1; };
-template <> struct OpT<SKF_HELMHOLTZ_ADL_3D> { static constexpr int dim = 3, cplx = 1, geom = 2; };
-
-// K(x, y) for dx = x - y and the normal dot product rn = dx . n (source or target normal).
-template <typename R, int OP>
-__device__ inline Cplx<R> kernel_value(R dx, R dy, R dz, R rn, R kappa) {
-  using M = Mth<R>;
-  constexpr R inv4pi = R(0.079577471545947667884441881686257181);
-  Cplx<R> k;
-  R r2;
-  if (M::fp32_bias) {
-    // FP32 form: r^2 is accumulated on top of the clamp (3 FMAs, no separate max)
-    r2 = fma(dx, dx, M::clamp_cd);
-    r2 = fma(dy, dy, r2);
-    if (OpT<OP>::dim == 3) r2 = fma(dz, dz, r2);
-  } else {
-    r2 = dx * dx + dy * dy;
-    if (OpT<OP>::dim == 3) r2 += dz * dz;
-  }
-  if (OP == SKF_LAPLACE_2D) {
-    k.re = M::half_lg(M::fp32_bias ? r2 : fmax(r2, M::clamp_cd));
-    k.im = 0;
-  } else if (OP == SKF_LAPLACE_3D) {
-    k.re = M::rsq(M::fp32_bias ? r2 : fmax(r2, M::clamp_cd));
-    k.im = 0;
-  } else if (OP == SKF_HELMHOLTZ_2D) {
-    R r = sqrt(M::fp32_bias ? r2 : fmax(r2, M::clamp_cd));
-    k.re = M::bj0(kappa * r);
-    k.im = M::by0(kappa * r);
-  } else if (OP == SKF_HELMHOLTZ_3D) {
-    const R rc = M::fp32_bias ? r2 : fmax(r2, M::clamp_cd);
-    R ir = M::rsq(rc);
-    R r = rc * ir;
-    R s, c;
-    M::sc(kappa * r, &s, &c);
-    k.re = c * ir;
-    k.im = s * ir;
-  } else if (OP == SKF_LAPLACE_DL_3D || OP == SKF_LAPLACE_ADL_3D) {
-    R ir = M::rsq(M::fp32_bias ? r2 : fmax(r2, M::clamp_dl));
-    R v = inv4pi * rn * ir * ir * ir;
-    k.re = (OP == SKF_LAPLACE_DL_3D) ? -v : v;
-    k.im = 0;
-  } else {  // Helmholtz double layers: (1/4pi) rn e^{ikr} (ikr - 1) / r^3, sign flipped for the adjoint
-    R rc = M::fp32_bias ? r2 : fmax(r2, M::clamp_cd);
-    R ir = M::rsq(rc);
-    R r = rc * ir;
-    R s, c;
-    R kr = kappa * r;
-    M::sc(kr, &s, &c);
-    R f = inv4pi * rn * ir * ir * ir;
-    if (OP == SKF_HELMHOLTZ_ADL_3D) f = -f;
-    // e^{ikr}(ikr - 1) = (c + i s)(-1 + i kr) = (-c - kr s) + i (kr c - s)
-    k.re = f * (-c - kr * s);
-    k.im = f * (kr * c - s);
-  }
-  return k;
-}
-
-// --------------------------------------------------------------- P2P kernel
 namespace {
-
-constexpr int kWG = 256;   // 4 wave64 per workgroup
-constexpr int kTCH = 64;   // targets per work item (one per lane)
-
-struct P2PDev {
-  const void* px;
-  const void* py;
-  const void* pz;
-  const void* nx;
-  const void* ny;
-  const void* nz;
-  const double* centers;
-  const void* q;
-  void* u;
-  const int* g_pt;
-  const int* g_n;
-  const int* g_out;
-  const int* g_box;
-  const int* e_pt;
-  const int* e_n;
-  const int* e_q;
-  const int* e_box;
-  const int* e_flags;
-  const int4* items;
-  int width;
-  int nch;  // channels (complex channels for complex kernels)
-  double kappa;
-  double* urev;  // reverse outputs of the symmetric kernel (FP64; u itself in the FP64 form)
-};
-
-__device__ inline void atomic_acc(double* p, double v) { unsafeAtomicAdd(p, v); }
-__device__ inline void atomic_acc(float* p, float v) { unsafeAtomicAdd(p, v); }
-
-// Values per staged source: coordinates, optional source normal, NCH charges (x2 if complex).
-template <int OP, int NCH>
-struct Stage {
-  static constexpr int dim = OpT<OP>::dim;
-  static constexpr int cw = OpT<OP>::cplx ? 2 : 1;
-  static constexpr int ng = OpT<OP>::geom == 1 ? 3 : 0;
-  static constexpr int nraw = dim + ng + NCH * cw;
-  static constexpr int nv = (nraw + 1) & ~1;  // even -> 16-byte aligned rows
-};
-
-// Sum over `cnt` staged sources of one wave's tile for this lane's target. SELF skips the
-// source whose local index equals the target's (the self pair's zero diagonal).
-template <typename R, int OP, int NCH, bool SELF>
-__device__ inline void tile_sum(const R* __restrict__ tw, int cnt, int self_k, R xs0, R ys0, R zs0, R gxt, R gyt,
-                                R gzt, R kappa, R* part) {
-  using S = Stage<OP, NCH>;
-  constexpr int NV = S::nv;
-  constexpr int CW = S::cw;
-#pragma unroll 4
-  for (int k = 0; k < cnt; ++k) {
-    const R* row = tw + k * NV;
-    R dx = xs0 - row[0];
-    R dy = ys0 - row[1];
-    R dz = (S::dim == 3) ? zs0 - row[2] : R(0);
-    R rn = 0;
-    if (OpT<OP>::geom == 1) rn = dx * row[S::dim] + dy * row[S::dim + 1] + dz * row[S::dim + 2];
-    if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
-    Cplx<R> kv = kernel_value<R, OP>(dx, dy, dz, rn, kappa);
-    if (SELF && k == self_k) {
-      kv.re = 0;
-      kv.im = 0;
-    }
-    const R* qq = row + S::dim + S::ng;
-    if (CW == 1) {
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) part[c] += kv.re * qq[c];
-    } else {
-#pragma unroll
-      for (int c = 0; c < NCH; ++c) {
-        part[2 * c] += kv.re * qq[2 * c] - kv.im * qq[2 * c + 1];
-        part[2 * c + 1] += kv.re * qq[2 * c + 1] + kv.im * qq[2 * c];
-      }
-    }
-  }
-}
-
-// One work item: up to 64 targets of one group against a range of its source entries.
-// The 4 waves of the workgroup split each 256-source tile (64 sources per wave);
-// partial sums are combined through LDS and added to the output with atomics.
-template <typename R, int OP, int NCH, bool PREC>
-__global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
-  using S = Stage<OP, NCH>;
-  constexpr int NV = S::nv;
-  constexpr int CW = S::cw;
-  constexpr int NA = NCH * CW;
-  __shared__ R tile[kWG * NV];
-  __shared__ double red[4][kTCH][NA];
-
-  const int4 it = d.items[blockIdx.x];
-  const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
-  const int ch0 = blockIdx.y * NCH;
-  const int nchv = min(NCH, d.nch - ch0);
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int ng = d.g_n[g];
-  const int nt = min(kTCH, ng - t0);
-  const int tloc = t0 + lane;
-  const bool tvalid = lane < nt;
-  const int tslot = d.g_pt[g] + (tvalid ? tloc : t0);
-
-  const R* px = static_cast<const R*>(d.px);
-  const R* py = static_cast<const R*>(d.py);
-  const R* pz = static_cast<const R*>(d.pz);
-  const R* qv = static_cast<const R*>(d.q);
-  R xt = px[tslot], yt = py[tslot], zt = (S::dim == 3) ? pz[tslot] : R(0);
-  R gxt = 0, gyt = 0, gzt = 0;
-  if (OpT<OP>::geom == 2) {
-    gxt = static_cast<const R*>(d.nx)[tslot];
-    gyt = static_cast<const R*>(d.ny)[tslot];
-    gzt = static_cast<const R*>(d.nz)[tslot];
-  }
-  double ctx = 0, cty = 0, ctz = 0;
-  if (PREC) {
-    const int gb = d.g_box[g];
-    ctx = d.centers[3 * gb + 0];
-    cty = d.centers[3 * gb + 1];
-    ctz = d.centers[3 * gb + 2];
-  }
-  const R kappa = R(d.kappa);
-
-  double acc[NA];
-#pragma unroll
-  for (int a = 0; a < NA; ++a) acc[a] = 0.0;
-
-  for (int s = sb; s < se; ++s) {
-    const int ps = d.e_pt[s], ns = d.e_n[s], qo = d.e_q[s], fl = d.e_flags[s];
-    const bool self = (fl & SKF_SRC_SELF) != 0;
-    const double sgn = (fl & SKF_SRC_NEG) ? -1.0 : 1.0;
-    // target coordinates relative to this source box (precision form)
-    R xs0 = xt, ys0 = yt, zs0 = zt;
-    if (PREC) {
-      const int eb = d.e_box[s];
-      xs0 = xt + R(ctx - d.centers[3 * eb + 0]);
-      ys0 = yt + R(cty - d.centers[3 * eb + 1]);
-      zs0 = zt + R(ctz - d.centers[3 * eb + 2]);
-    }
-    for (int base = 0; base < ns; base += kWG) {
-      // stage 256 sources (coordinates, normals, charges of this channel block)
-      {
-        const int j = base + tid;
-        R* row = &tile[tid * NV];
-        if (j < ns) {
-          const int sl = ps + j;
-          row[0] = px[sl];
-          row[1] = py[sl];
-          if (S::dim == 3) row[2] = pz[sl];
-          if (S::ng) {
-            row[S::dim + 0] = static_cast<const R*>(d.nx)[sl];
-            row[S::dim + 1] = static_cast<const R*>(d.ny)[sl];
-            row[S::dim + 2] = static_cast<const R*>(d.nz)[sl];
-          }
-          const R* qs = qv + (long long)(qo + j) * d.width + ch0 * CW;
-#pragma unroll
-          for (int c = 0; c < NA; ++c) row[S::dim + S::ng + c] = (c < nchv * CW) ? qs[c] : R(0);
-        } else {
-#pragma unroll
-          for (int c = 0; c < NV; ++c) row[c] = R(0);
-        }
-      }
-      __syncthreads();
-      const int wb = base + wave * 64;  // wave-uniform
-      const int cnt = __builtin_amdgcn_readfirstlane(max(0, min(64, ns - wb)));
-      R part[NA];
-#pragma unroll
-      for (int a = 0; a < NA; ++a) part[a] = 0;
-      const R* tw = &tile[wave * 64 * NV];
-      // the diagonal of a self segment falls in this tile only for targets tloc in [wb, wb + cnt)
-      if (self && tloc - wb >= 0 && tloc - wb < 64)
-        tile_sum<R, OP, NCH, true>(tw, cnt, tloc - wb, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part);
-      else
-        tile_sum<R, OP, NCH, false>(tw, cnt, -1, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part);
-#pragma unroll
-      for (int a = 0; a < NA; ++a) acc[a] += sgn * double(part[a]);
-      __syncthreads();
-    }
-  }
-#pragma unroll
-  for (int a = 0; a < NA; ++a) red[wave][lane][a] = acc[a];
-  __syncthreads();
-  if (wave == 0 && tvalid) {
-    R* uo = static_cast<R*>(d.u) + (long long)(d.g_out[g] + tloc) * d.width + ch0 * CW;
-#pragma unroll
-    for (int a = 0; a < NA; ++a) {
-      if (a < nchv * CW) {
-        double v = red[0][lane][a] + red[1][lane][a] + red[2][lane][a] + red[3][lane][a];
-        atomic_acc(&uo[a], R(v));
-      }
-    }
-  }
-}
 
 // ------------------------------------------- symmetric near field (single-layer kernels)
 // For two point sets U, V that interact in both directions with the same sign and a symmetric
@@ -433,7 +126,7 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
     const int ps = d.e_pt[s], ns = d.e_n[s], qo = d.e_q[s], fl = d.e_flags[s];
     const double sgn = (fl & SKF_SRC_NEG) ? -1.0 : 1.0;
     // precision form: sources are shifted into the target box's frame (minus the difference of
-    // the centres), one shift per source for all of a lane's targets
+    // the centres) as they are staged, once per source for all targets of the item
     R ox = 0, oy = 0, oz = 0;
     if (PREC) {
       const int eb = d.e_box[s];
@@ -447,9 +140,9 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
         R* row = &tile[tid * NV];
         if (j < ns) {
           const int sl = ps + j;
-          row[0] = px[sl];
-          row[1] = py[sl];
-          if (S::dim == 3) row[2] = pz[sl];
+          row[0] = PREC ? px[sl] - ox : px[sl];
+          row[1] = PREC ? py[sl] - oy : py[sl];
+          if (S::dim == 3) row[2] = PREC ? pz[sl] - oz : pz[sl];
           const R* qs = qv + (long long)(qo + j) * w;
 #pragma unroll
           for (int c = 0; c < CW; ++c) row[S::dim + c] = qs[c];
@@ -471,9 +164,9 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
 #pragma unroll UNR
         for (int jj = 0; jj < nj; ++jj) {
           const R* row = tw + (sub + jj) * NV;
-          const R xs = PREC ? row[0] - ox : row[0];
-          const R ys = PREC ? row[1] - oy : row[1];
-          const R zs = (S::dim == 3) ? (PREC ? row[2] - oz : row[2]) : R(0);
+          const R xs = row[0];
+          const R ys = row[1];
+          const R zs = (S::dim == 3) ? row[2] : R(0);
           R r0 = 0, r1 = 0;
 #pragma unroll
           for (int k = 0; k < TPL; ++k) {
@@ -896,7 +589,24 @@ bool op_is_complex(int op) {
   return op == SKF_HELMHOLTZ_2D || op == SKF_HELMHOLTZ_3D || op == SKF_HELMHOLTZ_DL_3D ||
          op == SKF_HELMHOLTZ_ADL_3D;
 }
-bool op_dim3(int op) { return !(op == SKF_LAPLACE_2D || op == SKF_HELMHOLTZ_2D); }
+bool op_dim3(int op) { return !(op == SKF_LAPLACE_2D || op == SKF_HELMHOLTZ_2D || op == SKF_USER_2D); }
+bool op_user(int op) { return op == SKF_USER_2D || op == SKF_USER_3D; }
+
+// near field of a user-defined operator: p2p_kernel instances compiled at run time (hiprtc,
+// skf_jit_compile) for channel blocks of 1, 2, 4 and 8, launched through the module API with
+// the same argument block as the library's own instances
+int launch_user(const skf_p2p_args* a, const P2PDev& d, int nitems, hipStream_t st) {
+  static const int nchb[4] = {1, 2, 4, 8};
+  const int k = d.nch == 1 ? 0 : d.nch <= 2 ? 1 : d.nch <= 4 ? 2 : 3;
+  hipFunction_t fn = static_cast<hipFunction_t>(a->user_fn[k]);
+  if (!fn) return set_err(SKF_ERR_ARG, "no run-time compiled near-field kernel for blocks of %d channels", nchb[k]);
+  const unsigned gy = unsigned((d.nch + nchb[k] - 1) / nchb[k]);
+  if (gy > 65535u) return set_err(SKF_ERR_ARG, "too many channels");
+  P2PDev dd = d;
+  void* args[] = {&dd};
+  SKF_HIP(hipModuleLaunchKernel(fn, unsigned(nitems), gy, 1, kWG, 1, 1, 0, st, args, nullptr));
+  return SKF_OK;
+}
 bool op_geom(int op) { return op >= SKF_LAPLACE_DL_3D && op <= SKF_HELMHOLTZ_ADL_3D; }
 
 int run_p2p(const skf_p2p_args* a, const void* q, void* u, hipStream_t st) {
@@ -921,6 +631,7 @@ int run_p2p(const skf_p2p_args* a, const void* q, void* u, hipStream_t st) {
   d.urev = nullptr;
   const int mfma_min = a->mfma_min_channels == 0 ? 16 : a->mfma_min_channels;
   if (a->precision != 0 && a->precision != 1) return set_err(SKF_ERR_ARG, "precision must be 0 or 1");
+  if (op_user(a->kernel)) return launch_user(a, d, a->nitems, st);
   const bool mfma = !cx && mfma_min > 0 && d.nch >= mfma_min;
   const bool sym = a->one && a->sym && d.nch == 1 && op_symmetric(a->kernel) && !mfma &&
                    (a->precision == 0 || (a->urev && a->urev_len > 0));
@@ -1669,4 +1380,87 @@ extern "C" int skf_p2p_blocks(int kernel, int precision, int width, double kappa
   if (precision == 1)
     return launch_blocks<float>(kernel, nblocks, m, n, ndim, xt, xs, nt, ns, q, u, width, self_interaction, kappa, st);
   return set_err(SKF_ERR_ARG, "precision must be 0 or 1");
+}
+
+// ------------------------------------------------- run-time compiled kernels
+// Near-field kernels of user-defined operators: hiprtc compiles a source that includes
+// p2p_core.h (passed in as a named header) with the operator's OpT<> / kernel_value<>
+// specializations, for gfx950 with the library's own code-generation flags; the requested
+// template instances are resolved by name expression and loaded as one module.
+#include <hip/hiprtc.h>
+
+static std::string g_jit_log;
+
+extern "C" const char* skf_jit_log(void) { return g_jit_log.c_str(); }
+
+extern "C" int skf_jit_compile(const char* source, const char* header_name, const char* header_source,
+                               const char* const* names, int nnames, void** module, void** functions,
+                               long long* code_bytes) {
+  if (!source || nnames < 0 || (nnames > 0 && !names)) return set_err(SKF_ERR_ARG, "bad jit arguments");
+  if (module && !functions && nnames > 0) return set_err(SKF_ERR_ARG, "functions array needed");
+  g_jit_log.clear();
+  hiprtcProgram prog;
+  const char* hsrc[1] = {header_source};
+  const char* hname[1] = {header_name};
+  const int nh = (header_name && header_source) ? 1 : 0;
+  if (hiprtcCreateProgram(&prog, source, "skf_user_p2p.hip", nh, hsrc, hname) != HIPRTC_SUCCESS)
+    return set_err(SKF_ERR_HIP, "hiprtcCreateProgram failed");
+  for (int i = 0; i < nnames; ++i) {
+    if (hiprtcAddNameExpression(prog, names[i]) != HIPRTC_SUCCESS) {
+      hiprtcDestroyProgram(&prog);
+      return set_err(SKF_ERR_ARG, "bad kernel name expression %s", names[i]);
+    }
+  }
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-munsafe-fp-atomics",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult cr = hiprtcCompileProgram(prog, int(sizeof(opts) / sizeof(opts[0])), opts);
+  size_t ls = 0;
+  if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
+    g_jit_log.assign(ls, '\0');
+    hiprtcGetProgramLog(prog, &g_jit_log[0]);
+    g_jit_log.resize(strlen(g_jit_log.c_str()));
+  }
+  if (cr != HIPRTC_SUCCESS) {
+    hiprtcDestroyProgram(&prog);
+    return set_err(SKF_ERR_ARG, "run-time compilation failed (skf_jit_log holds the compiler log)");
+  }
+  size_t cs = 0;
+  if (hiprtcGetCodeSize(prog, &cs) != HIPRTC_SUCCESS || cs == 0) {
+    hiprtcDestroyProgram(&prog);
+    return set_err(SKF_ERR_HIP, "no code from run-time compilation");
+  }
+  if (code_bytes) *code_bytes = (long long)cs;
+  if (!module) {  // compile-only check (no device needed)
+    hiprtcDestroyProgram(&prog);
+    return SKF_OK;
+  }
+  std::vector<char> code(cs);
+  std::vector<std::string> lowered(nnames);
+  bool ok = hiprtcGetCode(prog, code.data()) == HIPRTC_SUCCESS;
+  for (int i = 0; ok && i < nnames; ++i) {
+    const char* ln = nullptr;
+    ok = hiprtcGetLoweredName(prog, names[i], &ln) == HIPRTC_SUCCESS && ln;
+    if (ok) lowered[i] = ln;
+  }
+  hiprtcDestroyProgram(&prog);
+  if (!ok) return set_err(SKF_ERR_HIP, "run-time compilation: code or kernel names unavailable");
+  hipModule_t mod;
+  SKF_HIP(hipModuleLoadData(&mod, code.data()));
+  for (int i = 0; i < nnames; ++i) {
+    hipFunction_t fn;
+    const hipError_t e = hipModuleGetFunction(&fn, mod, lowered[i].c_str());
+    if (e != hipSuccess) {
+      (void)hipModuleUnload(mod);
+      return set_err(SKF_ERR_HIP, "hipModuleGetFunction(%s): %s", names[i], hipGetErrorString(e));
+    }
+    functions[i] = reinterpret_cast<void*>(fn);
+  }
+  *module = reinterpret_cast<void*>(mod);
+  return SKF_OK;
+}
+
+extern "C" int skf_jit_unload(void* module) {
+  if (!module) return SKF_OK;
+  SKF_HIP(hipModuleUnload(static_cast<hipModule_t>(module)));
+  return SKF_OK;
 }

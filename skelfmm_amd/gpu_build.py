@@ -19,7 +19,8 @@ Two paths for the boxes of one tree level, both in FP64 (complex128 for Helmholt
   predicted full rank keep all their points without a factorization.
 
 Results are returned in the CPU build's convention: ``(rank, idx, proj)`` with
-``idx`` the column pivots and ``proj = T``.
+``idx`` the column pivots (host) and ``proj = T``, which stays in device memory
+(``DeviceProj``) until the box layout gathers it there.
 """
 
 from __future__ import annotations
@@ -33,7 +34,7 @@ import torch
 from . import hip
 from .proxy import get_cube_surface, get_sphere_surface
 
-__all__ = ["GPUSkeletonizer", "accept_sketch"]
+__all__ = ["GPUSkeletonizer", "DeviceProj", "accept_sketch"]
 
 LDS_BYTES = 64 * 1024
 
@@ -72,6 +73,41 @@ def accept_sketch(Z, result, lrows, tol, cap=None, accept=6.0):
     return float(np.max(np.linalg.norm(E, axis=0))) <= accept * tol * max(den, 1e-300)
 
 
+class DeviceProj:
+    """Interpolation matrix ``T_b`` (``rank x (n - rank)``, row-major) left in device memory.
+
+    ``buf`` is the flat FP64 result buffer of one batched TRSM launch (complex values as
+    interleaved pairs) and ``off`` the box's first element in it. The box layout gathers
+    these on the device (``layout.build_layout``); ``numpy.asarray(p)`` copies one matrix
+    to the host for code that wants it there."""
+
+    __slots__ = ("buf", "off", "shape", "dtype")
+
+    def __init__(self, buf, off, shape, cplx):
+        self.buf = buf
+        self.off = int(off)
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.dtype = np.dtype(np.complex128 if cplx else np.float64)
+
+    @property
+    def size(self):
+        return self.shape[0] * self.shape[1]
+
+    @property
+    def nbytes(self):
+        return self.size * self.dtype.itemsize
+
+    @property
+    def ndim(self):
+        return 2
+
+    def __array__(self, dtype=None, copy=None):
+        w = 2 if self.dtype.kind == "c" else 1
+        h = self.buf[self.off * w:(self.off + self.size) * w].cpu().numpy()
+        a = (h.view(np.complex128) if w == 2 else h).reshape(self.shape)
+        return a if dtype is None else a.astype(dtype)
+
+
 class GPUSkeletonizer:
     """Batched GPU IDs for one build (operator, kappa, tolerance, points, normals)."""
 
@@ -96,7 +132,8 @@ class GPUSkeletonizer:
         self.gen.manual_seed(seed)
         self.timings = {}
         self.sketch_stats = {}  # rejected sketches by reason: rows_exhausted (rank too low), residual
-        self.profile = False  # True: synchronize and time every phase (proxy, gemm, qr, check, host)
+        self.profile = False  # True: synchronize and time every phase (proxy, gemm, qr, check, d2h, host)
+        self.device_proj = True  # interpolation matrices stay on the device (DeviceProj)
         self._t = None
 
     # ----------------------------------------------------------------- common
@@ -201,22 +238,34 @@ class GPUSkeletonizer:
         return d, host
 
     def _results(self, d, host, keep):
-        """Host (rank, idx, proj) per box where ``keep`` holds, else None."""
+        """(rank, idx, proj) per box where ``keep`` holds, else None.
+
+        ``idx`` (the column pivots) comes to the host; ``proj`` stays on the device as a
+        ``DeviceProj`` (or a host array with ``device_proj=False``)."""
         perm_h = d["perm"].cpu().numpy().astype(np.int64)
-        T_h = d["T"].cpu().numpy()
-        if self.cplx:
-            T_h = T_h.view(np.complex128)
+        T_h = None
+        if not self.device_proj:
+            T_h = d["T"].cpu().numpy()
+            if self.cplx:
+                T_h = T_h.view(np.complex128)
+        self._phase("d2h")
         out = []
-        for i in range(len(host["ncols"])):
+        ncols, ranks, p_off, t_off = host["ncols"], host["rank"], host["p_off"], host["t_off"]
+        empty = np.complex128 if self.cplx else np.float64
+        for i in range(len(ncols)):
             if not keep[i]:
                 out.append(None)
                 continue
-            n, r = int(host["ncols"][i]), int(host["rank"][i])
-            po, to = host["p_off"][i], host["t_off"][i]
+            n, r = int(ncols[i]), int(ranks[i])
+            po, to = p_off[i], t_off[i]
             idx = perm_h[po:po + n]
-            proj = T_h[to:to + r * (n - r)].reshape(r, n - r) if r * (n - r) > 0 else \
-                np.zeros((r, n - r), dtype=np.complex128 if self.cplx else np.float64)
-            out.append((r, idx, np.ascontiguousarray(proj)))
+            if r * (n - r) == 0:
+                proj = np.zeros((r, n - r), dtype=empty)
+            elif T_h is None:
+                proj = DeviceProj(d["T"], to, (r, n - r), self.cplx)
+            else:
+                proj = T_h[to:to + r * (n - r)].reshape(r, n - r)
+            out.append((r, idx, proj))
         self._phase("host")
         return out
 

@@ -40,6 +40,9 @@ EXPORTED_SYMBOLS = (
     "skf_mlp_forward",
     "skf_mlp_backward",
     "skf_adam_step",
+    "skf_jit_compile",
+    "skf_jit_log",
+    "skf_jit_unload",
 )
 
 _vp = ctypes.c_void_p
@@ -63,7 +66,7 @@ class P2PArgs(ctypes.Structure):
         ("nentries", _i), ("e_pt", _vp), ("e_n", _vp), ("e_q", _vp), ("e_box", _vp), ("e_flags", _vp),
         ("nitems", _i), ("items", _vp), ("mfma_min_channels", _i),
         ("one", ctypes.POINTER(P2PLists)), ("sym", ctypes.POINTER(P2PLists)), ("urev", _vp),
-        ("urev_len", ctypes.c_longlong), ("sym_unroll", _i),
+        ("urev_len", ctypes.c_longlong), ("sym_unroll", _i), ("user_fn", _vp * 4),
     ]
 
 
@@ -119,8 +122,12 @@ def lib():
     L.skf_mlp_backward.argtypes = [_vp, _vp, _vp, _i, _vp, _vp, _vp]
     L.skf_adam_step.argtypes = [_vp, _vp, _vp, _vp, _i, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                 ctypes.c_float, _i, _vp]
+    L.skf_jit_compile.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p),
+                                  _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_longlong)]
+    L.skf_jit_log.restype = ctypes.c_char_p
+    L.skf_jit_unload.argtypes = [_vp]
     for name in EXPORTED_SYMBOLS:
-        if name not in ("skf_last_error", "skf_version", "skf_build_last_error"):
+        if name not in ("skf_last_error", "skf_version", "skf_build_last_error", "skf_jit_log"):
             getattr(L, name).restype = _i
     _LIB = L
     return L

@@ -124,18 +124,10 @@ def build_layout(fmm, max_sources=16384):
     L.leaf_user = np.concatenate(lu).astype(np.int32)
     assert L.leaf_slot.shape[0] == fmm.N and np.unique(L.leaf_user).shape[0] == fmm.N
 
-    # interpolation matrices, one ragged array
-    toff = np.zeros(nb, dtype=np.int64)
-    tparts = []
-    acc = 0
-    for b in range(nb):
-        p = fmm.proj[b]
-        toff[b] = acc
-        if p is not None and rk[b] > 0 and p.size > 0:
-            tparts.append(np.ascontiguousarray(p).ravel())
-            acc += p.size
-    L.T = np.concatenate(tparts) if tparts else np.zeros(1, dtype=fmm.kernel_dtype)
-    L.T = L.T.astype(fmm.kernel_dtype, copy=False)
+    # interpolation matrices, one ragged array; matrices a GPU build left in device memory
+    # (``gpu_build.DeviceProj``) are gathered there: the device array is the host-built part
+    # followed by each result buffer whole, and toff points into it
+    L.T, toff = _gather_T(fmm, nb, rk)
     L.toff = toff
 
     # upward stages: deepest level first; the root level only gathers
@@ -190,6 +182,44 @@ def build_layout(fmm, max_sources=16384):
     _near_field(L, fmm, off, bs, rk, max_sources)
     L.max_sources = int(max_sources)
     return L
+
+
+def _gather_T(fmm, nb, rk):
+    """(T, toff): every box's ``T`` in one array (numpy, or a device tensor when any is on the device)."""
+    toff = np.zeros(nb, dtype=np.int64)
+    tparts, acc = [], 0
+    bufs = {}
+    dev = []
+    for b in range(nb):
+        p = fmm.proj[b]
+        if p is None or rk[b] <= 0 or p.size == 0:
+            continue
+        if hasattr(p, "buf"):  # DeviceProj
+            bufs.setdefault(id(p.buf), p.buf)
+            dev.append((b, id(p.buf), p.off))
+            continue
+        toff[b] = acc
+        tparts.append(np.ascontiguousarray(p).ravel())
+        acc += p.size
+    host = np.concatenate(tparts) if tparts else np.zeros(0 if bufs else 1, dtype=fmm.kernel_dtype)
+    host = host.astype(fmm.kernel_dtype, copy=False)
+    if not bufs:
+        return host, toff
+    import torch
+
+    cplx = np.dtype(fmm.kernel_dtype).kind == "c"
+    w = 2 if cplx else 1
+    first = next(iter(bufs.values()))
+    parts = [torch.as_tensor(host.view(np.float64) if cplx else host).to(first.device)]
+    base, pos = {}, acc
+    for k, buf in bufs.items():
+        base[k] = pos
+        parts.append(buf)
+        pos += buf.numel() // w
+    for b, k, off in dev:
+        toff[b] = base[k] + off
+    T = torch.cat(parts)
+    return (torch.view_as_complex(T.view(-1, 2)) if cplx else T), toff
 
 
 def _empty_lists(tpi):
@@ -267,7 +297,13 @@ def merge_layouts(layouts):
     M.centers = np.concatenate([L.centers for L in layouts])
     M.leaf_slot = np.concatenate([L.leaf_slot + sb for L, sb in zip(layouts, slot_base)]).astype(np.int32)
     M.leaf_user = np.concatenate([L.leaf_user + pb for L, pb in zip(layouts, pt_base)]).astype(np.int32)
-    M.T = np.concatenate([L.T for L in layouts])
+    if any(not isinstance(L.T, np.ndarray) for L in layouts):  # some T on the device
+        import torch
+
+        dev = next(L.T.device for L in layouts if not isinstance(L.T, np.ndarray))
+        M.T = torch.cat([torch.as_tensor(L.T).to(dev) for L in layouts])
+    else:
+        M.T = np.concatenate([L.T for L in layouts])
     M.toff = np.concatenate([L.toff + tb for L, tb in zip(layouts, t_base)])
     M.root_level = min(L.root_level for L in layouts)
 

@@ -28,6 +28,7 @@ __all__ = [
     "HELMHOLTZ_DOUBLE_LAYER_3D",
     "HELMHOLTZ_ADJOINT_DOUBLE_LAYER_3D",
     "BUILTIN_OPERATORS",
+    "custom_operator",
 ]
 
 # operator codes shared with skelfmm_amd/csrc/skelfmm_hip.hip (enum skf_kernel)
@@ -63,6 +64,7 @@ class KernelOperator:
     is_symmetric: bool = False
     kernel_id: int = -1
     is_complex: bool = False
+    user_kernel: object = None  # jit.UserKernel of a custom_operator (run-time compiled near field)
 
     def __post_init__(self):
         if self.geometry_side not in {None, "source", "target"}:
@@ -170,6 +172,61 @@ BUILTIN_OPERATORS = (
     HELMHOLTZ_DOUBLE_LAYER_3D,
     HELMHOLTZ_ADJOINT_DOUBLE_LAYER_3D,
 )
+
+
+def _user_numpy(uk, XX, YY, param=0.0, assume_not_self_interaction=False):
+    """Dense ``K(x_i, y_j)`` of a user formula; zero where the points coincide."""
+    import numpy as np
+
+    XX = np.asarray(XX, dtype=np.float64)
+    YY = np.asarray(YY, dtype=np.float64)
+    if XX.shape[-1] != uk.ndim or YY.shape[-1] != uk.ndim:
+        raise ValueError(f"points must have {uk.ndim} coordinates")
+    diff, r2 = dense._diff_r2(XX, YY)
+    if assume_not_self_interaction:
+        return np.array(uk.numpy_values(diff, r2, param))
+    zero = r2 == 0
+    v = np.array(uk.numpy_values(diff, np.where(zero, 1.0, r2), param))
+    v[zero] = 0.0
+    return v
+
+
+def _user_torch(uk, q, XX, YY, param, self_interaction):
+    """``(B, m, G) = K(XX, YY) @ q`` for a user formula in torch (any device)."""
+    import torch
+
+    squeeze = XX.dim() == 2
+    if squeeze:
+        XX, YY, q = XX[None], YY[None], q[None]
+    diff = XX[..., :, None, :] - YY[..., None, :, :]
+    r2 = torch.sum(diff * diff, dim=-1)
+    zero = r2 == 0
+    K = uk.torch_values(diff, torch.where(zero, torch.ones_like(r2), r2), param)
+    K = torch.where(zero, torch.zeros_like(K), K)
+    if self_interaction:
+        m, n = K.shape[-2], K.shape[-1]
+        K = K.masked_fill(torch.eye(m, n, dtype=torch.bool, device=K.device), 0.0)
+    out = torch.matmul(K, q.to(K.dtype))
+    return out[0] if squeeze else out
+
+
+def custom_operator(name, expression, ndim=3, symmetric=None):
+    """A user-defined real point kernel ``K(x, y) = expression`` (``skelfmm_amd.jit`` language).
+
+    The host build evaluates it with numpy, ``torch_apply`` with torch, and the device apply
+    compiles its near field at run time (hiprtc) into the library's one-sided near-field
+    kernel. ``symmetric`` (default: the formula uses no ``dx dy dz``, so ``K(x, y) = K(y, x)``)
+    selects the single proxy block of the build, as for the built-in single layers."""
+    from . import jit
+
+    uk = jit.UserKernel(expression, ndim)
+    if symmetric is None:
+        symmetric = not ({"dx", "dy", "dz"} & uk.names_used)
+    op = PointKernelOperator(name=str(name), ndim=int(ndim), numpy_func=partial(_user_numpy, uk),
+                             torch_func=partial(_user_torch, uk), is_symmetric=bool(symmetric),
+                             kernel_id=uk.kernel_id)
+    op.user_kernel = uk
+    return op
 
 
 def _require_operator(operator):

@@ -77,7 +77,7 @@ def save_layout(path, L, *, operator_name, kappa, tol, leaf_size, points, geomet
     for k in _LAYOUT_ARRAYS:
         v = getattr(L, k, None)
         if v is not None:
-            arrays["layout/" + k] = np.asarray(v)
+            arrays["layout/" + k] = v.cpu().numpy() if hasattr(v, "cpu") else np.asarray(v)
     for name in ("up", "down"):
         for i, S in enumerate(getattr(L, name)):
             header[name].append({"level": int(S.level)})

@@ -48,8 +48,7 @@ class FMMOperator:
         self.dev = dev
         self.N = int(dev.N)
         self.device = dev.device
-        complex_op = dev.operator.is_complex or dev.kappa > 0
-        self.dtype = dev.complex_dtype if complex_op else dev.real_dtype
+        self.dtype = dev.complex_dtype if dev.complex_vectors else dev.real_dtype
         self.weights = None if weights is None else torch.as_tensor(weights, dtype=dev.real_dtype,
                                                                     device=self.device)
         if diag is None or np.isscalar(diag):

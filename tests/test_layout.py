@@ -197,3 +197,44 @@ def test_merged_layout_split_emulates_every_instance():
     q = rng.normal(size=L.N)
     np.testing.assert_allclose(emulate_apply(L, "laplace_2d", 0.0, q, split=True),
                                emulate_apply(L, "laplace_2d", 0.0, q), rtol=0, atol=1e-11 * np.abs(q).sum())
+
+
+@pytest.mark.parametrize("case", [c for c in BUILDS if c[0] in ("lap2d", "annulus")], ids=lambda c: c[0])
+def test_device_resident_interpolation_matrices(case):
+    """T matrices held as slices of result buffers (the GPU build's DeviceProj, here on CPU
+    tensors, in shuffled order across two buffers with gaps) gather into the same layout."""
+    import copy
+
+    import torch
+
+    from skelfmm_amd.gpu_build import DeviceProj
+
+    _, f, k = case
+    L0 = build_layout(f, max_sources=900)
+    g = copy.copy(f)
+    g.proj = list(f.proj)
+    boxes = [b for b in range(f.nboxes) if f.proj[b] is not None and f.proj[b].size > 0]
+    assert len(boxes) > 10
+    rng = np.random.default_rng(0)
+    order = rng.permutation(len(boxes))
+    halves = [order[: len(order) // 2], order[len(order) // 2:]]
+    for part in halves:
+        sizes = [f.proj[boxes[i]].size + 3 for i in part]  # 3 unused entries after each matrix
+        buf = torch.zeros(int(sum(sizes)), dtype=torch.float64)
+        pos = 0
+        for i, sz in zip(part, sizes):
+            b = boxes[i]
+            p = f.proj[b]
+            buf[pos:pos + p.size] = torch.as_tensor(p.ravel())
+            g.proj[b] = DeviceProj(buf, pos, p.shape, cplx=False)
+            pos += sz
+    some = boxes[0]
+    np.testing.assert_array_equal(np.asarray(g.proj[some]), f.proj[some])
+    L1 = build_layout(g, max_sources=900)
+    assert torch.is_tensor(L1.T)
+    for b in boxes:
+        r, n = f.proj[b].shape
+        np.testing.assert_array_equal(L1.T[L1.toff[b]:L1.toff[b] + r * n].numpy().reshape(r, n), f.proj[b])
+    L1.T = L1.T.numpy()
+    q = np.random.RandomState(3).randn(f.N)
+    np.testing.assert_array_equal(emulate_apply(L1, f.operator.name, k, q), emulate_apply(L0, f.operator.name, k, q))
