@@ -321,8 +321,15 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ng = d.g_n[g];
-  const bool wave_busy = t0 + wave * 16 < ng;     // waves without targets only stage
-  const int tloc = t0 + wave * 16 + (lane & 15);  // target of this lane's A rows
+  // target blocks of 16 per item: with at most 32 targets the waves split the sources as well
+  // (TB blocks x SPL source parts; wave = part * TB + block), so small groups keep all four
+  // waves on matrix work; with more, one block per wave and waves past the last block only stage
+  const int nti = min(64, ng - t0);
+  const int TB = nti <= 16 ? 1 : nti <= 32 ? 2 : 4;
+  const int SPL = 4 / TB;
+  const int tb = wave % TB, spart = wave / TB;
+  const bool wave_busy = t0 + tb * 16 < ng;
+  const int tloc = t0 + tb * 16 + (lane & 15);  // target of this lane's A rows
   const bool tvalid = tloc < ng;
   const int tslot = d.g_pt[g] + (tvalid ? tloc : t0);
   const R* px = static_cast<const R*>(d.px);
@@ -429,7 +436,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) D[nb] = V{0, 0, 0, 0};
       const int ksteps = (nsrc + 3) >> 2;
-      for (int kk = 0; kk < ksteps; ++kk) {
+      for (int kk = spart; kk < ksteps; kk += SPL) {
         const int sj = kk * 4 + (lane >> 4);
         const R* row = &pt[sj * 8];
         const R dx = xt - row[0], dy = yt - row[1], dz = (DIM == 3) ? zt - row[2] : R(0);
@@ -462,8 +469,8 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
     if (ch >= ncv) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int tl = t0 + wave * 16 + MF::row(lane, r);
-      if (tl < ng) atomic_acc(&uo[(long long)(d.g_out[g] + tl) * d.width + c0 + ch], R(acc[nb][r]));
+      const int tl = t0 + tb * 16 + MF::row(lane, r);
+      if (wave_busy && tl < ng) atomic_acc(&uo[(long long)(d.g_out[g] + tl) * d.width + c0 + ch], R(acc[nb][r]));
     }
   }
 }
