@@ -166,7 +166,8 @@ typedef struct skf_plan_desc {
   const int* leaf_slot;       /* N entries: slot of the user's point i ... */
   const int* leaf_user;       /* ... and its user index */
   const void* T;              /* interpolation matrices, Real (or complex Real pairs) */
-  void* qbuf;                 /* Real[2*nslots*width]: q_vec then q_skel */
+  void* qbuf;                 /* Real[2*nslots*width]: q_vec then q_skel (every slot the apply
+                                 reads is written by it first: no clearing needed) */
   void* ubuf;                 /* Real[2*nslots*width]: u_vec then u_skel */
   int nup;                    /* upward stages, deepest level first (root gather last) */
   const skf_level_args* up;

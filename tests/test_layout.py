@@ -238,3 +238,30 @@ def test_device_resident_interpolation_matrices(case):
     L1.T = L1.T.numpy()
     q = np.random.RandomState(3).randn(f.N)
     np.testing.assert_array_equal(emulate_apply(L1, f.operator.name, k, q), emulate_apply(L0, f.operator.name, k, q))
+
+
+@pytest.mark.parametrize("case", BUILDS, ids=lambda c: c[0])
+def test_every_charge_slot_read_is_written_first(case):
+    """The apply writes every slot of the charge buffer it reads before reading it, so the
+    buffer needs no zero fill per apply: leaf gather, then per upward stage (deepest first)
+    the children's skeleton charges into the parents' candidate slots and the boxes'
+    skeleton charges; the near field reads candidate and skeleton ranges of its entries."""
+    from skelfmm_amd.layout import merge_layouts
+
+    _, f, _ = case
+    L = build_layout(f, max_sources=900)
+    if case[0] == "annulus":  # merged instances keep the property
+        L = merge_layouts([L, build_layout(BUILDS[0][1], max_sources=900)])
+    ns = L.nslots
+    written = np.zeros(2 * ns, dtype=bool)
+    written[L.leaf_slot] = True
+    for S in L.up:  # deepest level first, as skf_apply runs them
+        assert written[ns + S.gather_src].all(), f"level {S.level}: gather reads unwritten skeleton slots"
+        written[S.gather_dst] = True
+        for off, bs, r in zip(S.box_off, S.box_bs, S.box_rank):
+            assert written[off:off + bs].all(), f"level {S.level}: box candidates read before written"
+            written[ns + off:ns + off + r] = True
+    lists = [L] + list(L.split or ())
+    for P in lists:
+        for q0, n in zip(P.e_q, P.e_n):
+            assert written[q0:q0 + n].all(), "near field reads an unwritten charge slot"
