@@ -314,6 +314,12 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   __shared__ R qt[64 * kQS];
   __shared__ R pt[64 * 8];   // x, y, z, nx, ny, nz per source (precision form: in the target box frame)
   __shared__ int pself[64];  // local index of a self-segment source (its diagonal target), else -1
+  // the item's entries (up to kEnt): stream offsets and per-entry data, so that finding the entry
+  // of a staged source is a walk through LDS rather than a chain of dependent global loads
+  constexpr int kEnt = sizeof(R) == 4 ? 256 : 128;
+  __shared__ int e_start[kEnt + 1];
+  __shared__ int e_meta[kEnt][3];  // charge offset, first point, flags
+  __shared__ R e_shift[kEnt][3];   // precision form: centre difference into the target box frame
   const int4 it = d.items[blockIdx.x];
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
   const int c0 = blockIdx.y * kMB;
@@ -360,35 +366,101 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   // boundaries (small skeleton entries fill a tile together). Each staged source carries its
   // entry's sign (folded into its charges), self flag and, in the precision form, the shift
   // into the target box's frame.
+  const int nent = se - sb;
+  const bool etab = nent <= kEnt;  // uniform
   int tot = 0;
-  for (int s = sb; s < se; ++s) tot += d.e_n[s];
-  int cs = sb, cj = 0;  // stream cursor of the tile start (uniform)
+  if (etab) {
+    // wave 0 scans the entry sizes (4 per lane), every thread fills the per-entry table
+    if (wave == 0) {
+      int n4[4], loc = 0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = lane * 4 + u;
+        n4[u] = e < nent ? d.e_n[sb + e] : 0;
+        loc += n4[u];
+      }
+      int inc = loc;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+      }
+      int run = inc - loc;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int e = lane * 4 + u;
+        if (e <= kEnt) e_start[e] = run;
+        run += n4[u];
+      }
+      if (lane == 63 && 4 * 64 <= kEnt) e_start[kEnt] = run;
+    }
+    for (int e = tid; e < nent; e += kWG) {
+      const int s = sb + e;
+      e_meta[e][0] = d.e_q[s];
+      e_meta[e][1] = d.e_pt[s];
+      e_meta[e][2] = d.e_flags[s];
+      R ox = 0, oy = 0, oz = 0;
+      if (PREC) {
+        const int eb = d.e_box[s];
+        ox = R(ctx - d.centers[3 * eb + 0]);
+        oy = R(cty - d.centers[3 * eb + 1]);
+        oz = R(ctz - d.centers[3 * eb + 2]);
+      }
+      e_shift[e][0] = ox;
+      e_shift[e][1] = oy;
+      e_shift[e][2] = oz;
+    }
+    __syncthreads();
+    tot = e_start[nent];
+  } else {
+    for (int s = sb; s < se; ++s) tot += d.e_n[s];
+  }
+  int cs = sb, cj = 0;  // stream cursor of the tile start (uniform; global path)
+  int ce = 0;           // entry of the tile start (LDS path)
   // staging of one tile: global loads into registers, then LDS stores (loading the next tile
   // during the matrix work was measured slower: the extra registers cost a wave per SIMD)
   const int sr = tid >> 2, cq = (tid & 3) * 16;
   R rq[16], rp[6];
   int rself = -1;
-  auto stage_load = [&](int nsrc) {
+  auto stage_load = [&](int tbase, int nsrc) {
     if (sr < nsrc) {
-      int s = cs, j = cj + sr;
-      while (j >= d.e_n[s]) {
-        j -= d.e_n[s];
-        ++s;
-      }
-      const int fl = d.e_flags[s];
-      const R sgn = (fl & SKF_SRC_NEG) ? R(-1) : R(1);
-      const R* src = qv + (long long)(d.e_q[s] + j) * d.width + c0 + cq;
-#pragma unroll
-      for (int c = 0; c < 16; ++c) rq[c] = (cq + c < ncv) ? sgn * src[c] : R(0);
-      if ((tid & 3) == 0) {
-        const int sl = d.e_pt[s] + j;
-        R ox = 0, oy = 0, oz = 0;
+      int s, j, eq, ept, fl;
+      R ox = 0, oy = 0, oz = 0;
+      if (etab) {
+        const int pos = tbase + sr;
+        int e = ce;
+        while (e_start[e + 1] <= pos) ++e;
+        j = pos - e_start[e];
+        s = sb + e;
+        eq = e_meta[e][0];
+        ept = e_meta[e][1];
+        fl = e_meta[e][2];
+        ox = e_shift[e][0];
+        oy = e_shift[e][1];
+        oz = e_shift[e][2];
+      } else {
+        s = cs;
+        j = cj + sr;
+        while (j >= d.e_n[s]) {
+          j -= d.e_n[s];
+          ++s;
+        }
+        eq = d.e_q[s];
+        ept = d.e_pt[s];
+        fl = d.e_flags[s];
         if (PREC) {
           const int eb = d.e_box[s];
           ox = R(ctx - d.centers[3 * eb + 0]);
           oy = R(cty - d.centers[3 * eb + 1]);
           oz = R(ctz - d.centers[3 * eb + 2]);
         }
+      }
+      const R sgn = (fl & SKF_SRC_NEG) ? R(-1) : R(1);
+      const R* src = qv + (long long)(eq + j) * d.width + c0 + cq;
+#pragma unroll
+      for (int c = 0; c < 16; ++c) rq[c] = (cq + c < ncv) ? sgn * src[c] : R(0);
+      if ((tid & 3) == 0) {
+        const int sl = ept + j;
         rp[0] = px[sl] - ox;
         rp[1] = py[sl] - oy;
         rp[2] = (DIM == 3) ? pz[sl] - oz : R(0);
@@ -406,10 +478,15 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
       rself = -1;
     }
     // advance the cursor by the tile (uniform)
-    cj += nsrc;
-    while (cs < se && cj >= d.e_n[cs]) {
-      cj -= d.e_n[cs];
-      ++cs;
+    if (etab) {
+      const int next = tbase + nsrc;
+      while (ce < nent && e_start[ce + 1] <= next) ++ce;
+    } else {
+      cj += nsrc;
+      while (cs < se && cj >= d.e_n[cs]) {
+        cj -= d.e_n[cs];
+        ++cs;
+      }
     }
   };
   auto stage_store = [&]() {
@@ -424,7 +501,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
     }
   };
   if (tot > 0) {
-    stage_load(min(64, tot));
+    stage_load(0, min(64, tot));
     stage_store();
   }
   for (int base = 0; base < tot; base += 64) {
@@ -458,7 +535,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
     }
     __syncthreads();
     if (more) {
-      stage_load(min(64, tot - base - 64));
+      stage_load(base + 64, min(64, tot - base - 64));
       stage_store();
     }
   }
@@ -1102,7 +1179,7 @@ __global__ __launch_bounds__(256) void up_gemm_kernel(const R* __restrict__ T, c
   }
 }
 
-// Downward: a wave owns a quarter of the item's columns and handles 4 of them at a time.
+// Downward: a wave owns a quarter of the item's columns and handles CG of them at a time.
 template <typename R, bool CX>
 __global__ __launch_bounds__(256) void down_gemm_kernel(const R* __restrict__ T, const int* __restrict__ box_off,
                                                         const int* __restrict__ box_bs,
@@ -1111,7 +1188,8 @@ __global__ __launch_bounds__(256) void down_gemm_kernel(const R* __restrict__ T,
                                                         const int2* __restrict__ items, R* __restrict__ uvec,
                                                         const R* __restrict__ uskel, int w) {
   constexpr int CWd = CX ? 2 : 1;
-  constexpr int CG = 4;
+  // columns per group: each skeleton row read feeds CG columns (real FP32: 16, else 8)
+  constexpr int CG = (sizeof(R) == 4 && !CX) ? 16 : 8;
   const int2 it = items[blockIdx.x];
   const int b = it.x;
   const int off = box_off[b], bs = box_bs[b], r = box_rank[b];
