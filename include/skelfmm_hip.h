@@ -141,6 +141,10 @@ typedef struct skf_p2p_args {
   /* kernel = SKF_USER_2D / SKF_USER_3D: run-time compiled near-field kernels (hipFunction_t from
      skf_jit_compile) for channel blocks of 1, 2, 4 and 8, in this argument's precision */
   void* user_fn[4];
+  /* an upper bound of every pair distance (e.g. the point cloud's diameter), 0 = unknown; with
+     it the FP32 form evaluates e^{i kappa r} with a branch-free reduced sine/cosine when
+     kappa * max_distance <= 65536 */
+  double max_distance;
 } skf_p2p_args;
 
 /* one level of the upward or downward interpolation pass */

@@ -194,6 +194,9 @@ class SkelFMMBatched(torch.nn.Module):
         self.one_interactions = L.split[0].interactions if self._split is not None else L.interactions
         self.nslots = L.nslots
         self.N = L.N
+        # bound of every pair distance: the diameter of the points' bounding box
+        P = np.asarray(L.points)
+        self.max_distance = float(np.linalg.norm(P.max(axis=1) - P.min(axis=1))) if P.size else 0.0
         for k in list(self._plans):
             self._destroy_plan(k)
         self._graphs = {}
@@ -244,6 +247,7 @@ class SkelFMMBatched(torch.nn.Module):
         a.nitems = int(nf["items"].shape[0] // 4)
         a.mfma_min_channels = int(self.mfma_min_channels)
         a.sym_unroll = int(self.sym_unroll)
+        a.max_distance = float(self.max_distance)
         uk = getattr(self.operator, "user_kernel", None)
         if uk is not None:  # user-defined operator: near field compiled at run time
             from .jit import compile_kernels

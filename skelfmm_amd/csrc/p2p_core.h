@@ -59,6 +59,35 @@ struct Mth<double> {
   __device__ static inline double by0(double x) { return y0(x); }
 };
 
+// sin and cos of an FP32 argument with |x| <= kTrigFastMax, without a branch: Cody-Waite
+// reduction x = n (pi/2) + y with pi/2 split in three FP32 parts (each product exact inside
+// the FMA), Taylor polynomials of sin (to y^9) and cos (to y^10) on |y| <= pi/4 (truncation
+// below 2e-9), and the quadrant of n. Used by the FP32 form when the host bounds the kernel
+// argument (kappa times the largest pair distance); other arguments use sincosf.
+constexpr float kTrigFastMax = 65536.0f;
+
+__device__ inline void sincos_reduced(float x, float* sp, float* cp) {
+  const float n = rintf(x * 0.636619772367581343f);
+  float y = fmaf(-n, 1.57079637050628662f, x);
+  y = fmaf(-n, -4.37113882867379300e-08f, y);
+  y = fmaf(-n, -1.71512451000588185e-15f, y);
+  const float z = y * y;
+  float ps = fmaf(z, 2.75573192239858907e-06f, -1.98412698412698413e-04f);
+  ps = fmaf(z, ps, 8.33333333333333333e-03f);
+  ps = fmaf(z, ps, -1.66666666666666667e-01f);
+  const float sn = fmaf(y * z, ps, y);
+  float pc = fmaf(z, -2.75573192239858907e-07f, 2.48015873015873016e-05f);
+  pc = fmaf(z, pc, -1.38888888888888889e-03f);
+  pc = fmaf(z, pc, 4.16666666666666667e-02f);
+  pc = fmaf(z, pc, -0.5f);
+  const float cs = fmaf(z, pc, 1.0f);
+  const int q = int(n);
+  const float sv = (q & 1) ? cs : sn;
+  const float cv = (q & 1) ? sn : cs;
+  *sp = (q & 2) ? -sv : sv;
+  *cp = ((q + 1) & 2) ? -cv : cv;
+}
+
 template <>
 struct Mth<float> {
   static constexpr bool fp32_bias = true;
@@ -86,7 +115,24 @@ template <> struct OpT<SKF_HELMHOLTZ_DL_3D> { static constexpr int dim = 3, cplx
 template <> struct OpT<SKF_HELMHOLTZ_ADL_3D> { static constexpr int dim = 3, cplx = 1, geom = 2; };
 
 // K(x, y) for dx = x - y and the normal dot product rn = dx . n (source or target normal).
+// FT: the FP32 form's Helmholtz arguments are bounded by kTrigFastMax (the host checks it and
+// selects these instances)
+template <typename R, bool FT>
+__device__ inline void sincos_k(R x, R* s, R* c) {
+  if constexpr (FT && sizeof(R) == 4)
+    sincos_reduced(x, s, c);
+  else
+    Mth<R>::sc(x, s, c);
+}
+
+// instances with the reduced sincos exist for the FP32 3D Helmholtz kernels only
 template <typename R, int OP>
+struct HasFastTrig {
+  static constexpr bool value = sizeof(R) == 4 && (OP == SKF_HELMHOLTZ_3D || OP == SKF_HELMHOLTZ_DL_3D ||
+                                                   OP == SKF_HELMHOLTZ_ADL_3D);
+};
+
+template <typename R, int OP, bool FT = false>
 __device__ inline Cplx<R> kernel_value(R dx, R dy, R dz, R rn, R kappa) {
   using M = Mth<R>;
   constexpr R inv4pi = R(0.079577471545947667884441881686257181);
@@ -116,7 +162,7 @@ __device__ inline Cplx<R> kernel_value(R dx, R dy, R dz, R rn, R kappa) {
     R ir = M::rsq(rc);
     R r = rc * ir;
     R s, c;
-    M::sc(kappa * r, &s, &c);
+    sincos_k<R, FT>(kappa * r, &s, &c);
     k.re = c * ir;
     k.im = s * ir;
   } else if (OP == SKF_LAPLACE_DL_3D || OP == SKF_LAPLACE_ADL_3D) {
@@ -130,7 +176,7 @@ __device__ inline Cplx<R> kernel_value(R dx, R dy, R dz, R rn, R kappa) {
     R r = rc * ir;
     R s, c;
     R kr = kappa * r;
-    M::sc(kr, &s, &c);
+    sincos_k<R, FT>(kr, &s, &c);
     R f = inv4pi * rn * ir * ir * ir;
     if (OP == SKF_HELMHOLTZ_ADL_3D) f = -f;
     // e^{ikr}(ikr - 1) = (c + i s)(-1 + i kr) = (-c - kr s) + i (kr c - s)
@@ -169,6 +215,7 @@ struct P2PDev {
   int nch;  // channels (complex channels for complex kernels)
   double kappa;
   double* urev;  // reverse outputs of the symmetric kernel (FP64; u itself in the FP64 form)
+  int fast_trig;  // FP32 form: every kernel argument kappa r is within kTrigFastMax
 };
 
 __device__ inline void atomic_acc(double* p, double v) { unsafeAtomicAdd(p, v); }
@@ -186,7 +233,7 @@ struct Stage {
 
 // Sum over `cnt` staged sources of one wave's tile for this lane's target. SELF skips the
 // source whose local index equals the target's (the self pair's zero diagonal).
-template <typename R, int OP, int NCH, bool SELF>
+template <typename R, int OP, int NCH, bool SELF, bool FT>
 __device__ inline void tile_sum(const R* __restrict__ tw, int cnt, int self_k, R xs0, R ys0, R zs0, R gxt, R gyt,
                                 R gzt, R kappa, R* part) {
   using S = Stage<OP, NCH>;
@@ -201,7 +248,7 @@ __device__ inline void tile_sum(const R* __restrict__ tw, int cnt, int self_k, R
     R rn = 0;
     if (OpT<OP>::geom == 1) rn = dx * row[S::dim] + dy * row[S::dim + 1] + dz * row[S::dim + 2];
     if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
-    Cplx<R> kv = kernel_value<R, OP>(dx, dy, dz, rn, kappa);
+    Cplx<R> kv = kernel_value<R, OP, FT>(dx, dy, dz, rn, kappa);
     if (SELF && k == self_k) {
       kv.re = 0;
       kv.im = 0;
@@ -223,7 +270,7 @@ __device__ inline void tile_sum(const R* __restrict__ tw, int cnt, int self_k, R
 // One work item: up to 64 targets of one group against a range of its source entries.
 // The 4 waves of the workgroup split each 256-source tile (64 sources per wave);
 // partial sums are combined through LDS and added to the output with atomics.
-template <typename R, int OP, int NCH, bool PREC>
+template <typename R, int OP, int NCH, bool PREC, bool FT = false>
 __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
   using S = Stage<OP, NCH>;
   constexpr int NV = S::nv;
@@ -312,9 +359,9 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
       const R* tw = &tile[wave * 64 * NV];
       // the diagonal of a self segment falls in this tile only for targets tloc in [wb, wb + cnt)
       if (self && tloc - wb >= 0 && tloc - wb < 64)
-        tile_sum<R, OP, NCH, true>(tw, cnt, tloc - wb, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part);
+        tile_sum<R, OP, NCH, true, FT>(tw, cnt, tloc - wb, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part);
       else
-        tile_sum<R, OP, NCH, false>(tw, cnt, -1, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part);
+        tile_sum<R, OP, NCH, false, FT>(tw, cnt, -1, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part);
 #pragma unroll
       for (int a = 0; a < NA; ++a) acc[a] += sgn * double(part[a]);
       __syncthreads();

@@ -86,7 +86,7 @@ __device__ inline void wave_sync_lds() {
 }
 
 // One pass of the symmetric kernel over targets tp .. tp + 64 TPL - 1 of the item (TPL per lane).
-template <typename R, int OP, bool PREC, int UNR, int TPL>
+template <typename R, int OP, bool PREC, int UNR, int TPL, bool FT>
 __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, R* __restrict__ rvl,
                                          double* __restrict__ red, int tp, int tend, int t0, int sb, int se,
                                          double ctx, double cty, double ctz) {
@@ -173,7 +173,7 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
             const R dx = xt[k] - xs;
             const R dy = yt[k] - ys;
             const R dz = (S::dim == 3) ? zt[k] - zs : R(0);
-            const Cplx<R> kv = kernel_value<R, OP>(dx, dy, dz, R(0), kappa);
+            const Cplx<R> kv = kernel_value<R, OP, FT>(dx, dy, dz, R(0), kappa);
             if (CW == 1) {
               part[k] += kv.re * row[S::dim];
               r0 += kv.re * qt[k];
@@ -229,7 +229,7 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
   }
 }
 
-template <typename R, int OP, bool PREC, int UNR>
+template <typename R, int OP, bool PREC, int UNR, bool FT = false>
 __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
   using S = Stage<OP, 1>;
   constexpr int NV = S::nv;
@@ -257,13 +257,13 @@ __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
   for (int tp = t0; tp < tend;) {
     const int tpl = min(TPL, (tend - tp + 63) / 64);
     if (tpl >= 4)
-      sym_pass<R, OP, PREC, UNR, 4>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+      sym_pass<R, OP, PREC, UNR, 4, FT>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
     else if (tpl == 3)
-      sym_pass<R, OP, PREC, UNR, 3>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+      sym_pass<R, OP, PREC, UNR, 3, FT>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
     else if (tpl == 2)
-      sym_pass<R, OP, PREC, UNR, 2>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+      sym_pass<R, OP, PREC, UNR, 2, FT>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
     else
-      sym_pass<R, OP, PREC, UNR, 1>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+      sym_pass<R, OP, PREC, UNR, 1, FT>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
     tp += 64 * tpl;
   }
 }
@@ -571,6 +571,12 @@ template <typename R, int OP, int NCH, bool PREC>
 int launch_p2p_t(const P2PDev& d, int nitems, hipStream_t st) {
   if constexpr (p2p_lds_bytes<R, OP, NCH>() <= 65536) {
     dim3 grid(nitems, (d.nch + NCH - 1) / NCH);
+    if constexpr (HasFastTrig<R, OP>::value) {
+      if (d.fast_trig) {
+        hipLaunchKernelGGL((p2p_kernel<R, OP, NCH, PREC, true>), grid, dim3(kWG), 0, st, d);
+        return SKF_OK;
+      }
+    }
     hipLaunchKernelGGL((p2p_kernel<R, OP, NCH, PREC>), grid, dim3(kWG), 0, st, d);
     return SKF_OK;
   } else {
@@ -633,7 +639,11 @@ int launch_p2p_sym_u(int op, const P2PDev& d, int nitems, hipStream_t st) {
       hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_HELMHOLTZ_2D, PREC, UNR>), grid, block, 0, st, d);
       break;
     case SKF_HELMHOLTZ_3D:
-      hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_HELMHOLTZ_3D, PREC, UNR>), grid, block, 0, st, d);
+      if (HasFastTrig<R, SKF_HELMHOLTZ_3D>::value && d.fast_trig)
+        hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_HELMHOLTZ_3D, PREC, UNR, HasFastTrig<R, SKF_HELMHOLTZ_3D>::value>),
+                           grid, block, 0, st, d);
+      else
+        hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_HELMHOLTZ_3D, PREC, UNR>), grid, block, 0, st, d);
       break;
     default: return set_err(SKF_ERR_ARG, "kernel %d has no symmetric form", op);
   }
@@ -713,6 +723,10 @@ int run_p2p(const skf_p2p_args* a, const void* q, void* u, hipStream_t st) {
   d.nch = cx ? a->width / 2 : a->width;
   d.kappa = a->kappa;
   d.urev = nullptr;
+  // the FP32 form's Helmholtz kernels take the branch-free reduced sincos when the host bounds
+  // every argument kappa r (kappa times the largest pair distance) within its range
+  d.fast_trig = (a->precision == 1 && a->max_distance > 0.0 &&
+                 std::fabs(a->kappa) * a->max_distance <= double(kTrigFastMax)) ? 1 : 0;
   const int mfma_min = a->mfma_min_channels == 0 ? 16 : a->mfma_min_channels;
   if (a->precision != 0 && a->precision != 1) return set_err(SKF_ERR_ARG, "precision must be 0 or 1");
   if (op_user(a->kernel)) return launch_user(a, d, a->nitems, st);

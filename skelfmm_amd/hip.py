@@ -67,6 +67,7 @@ class P2PArgs(ctypes.Structure):
         ("nitems", _i), ("items", _vp), ("mfma_min_channels", _i),
         ("one", ctypes.POINTER(P2PLists)), ("sym", ctypes.POINTER(P2PLists)), ("urev", _vp),
         ("urev_len", ctypes.c_longlong), ("sym_unroll", _i), ("user_fn", _vp * 4),
+        ("max_distance", ctypes.c_double),
     ]
 
 
