@@ -552,9 +552,314 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   }
 }
 
+// FP32 form of the matrix-core near field with the charge tiles staged by LDS-direct loads
+// (global_load_lds_dwordx4: lane l's 16 bytes land at the instruction's LDS base + 16 l) into
+// two LDS buffers: the loads of tile i + 1 are issued before the matrix work of tile i and
+// waited for after it, so the staging latency overlaps the matrix work without holding the
+// charges in registers. A 64 x 64 charge tile is stored unpadded, the 16-byte chunk q of row r
+// at chunk q ^ 4 (r & 3): the B-operand reads (rows 4 kk .. 4 kk + 3, 16 columns each) fall on
+// distinct banks. Staged charges are raw: the entry sign multiplies the kernel value, and rows
+// past the tile's sources load row 0 with sign 0. The two buffers are distinct arrays and the
+// tile loop is unrolled by two, so the compiler can tell the LDS reads of the current tile from
+// the loads in flight. Needs width % 4 == 0 and 16-byte aligned charges.
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+typedef __attribute__((address_space(1))) void* global_void_ptr;
+
+// vmcnt(0), other counters unconstrained (gfx9 s_waitcnt encoding)
+__device__ inline void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+template <int OP>
+__global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
+  static_assert(OpT<OP>::cplx == 0, "real kernels only");
+  using S = Stage<OP, 1>;
+  using MF = Mfma<float>;
+  typedef typename MF::V V;
+  constexpr int DIM = S::dim;
+  constexpr int NG = S::ng;
+  constexpr int PS = NG ? 8 : 4;  // per-source row: x, y, z, sign (+ normals)
+  constexpr int SG = NG ? 6 : 3;  // position of the sign in the row
+  __shared__ __attribute__((aligned(16))) float qa[64 * kMB];
+  __shared__ __attribute__((aligned(16))) float qb[64 * kMB];
+  __shared__ float pa[64 * PS];
+  __shared__ float pb[64 * PS];
+  __shared__ int sa[64];
+  __shared__ int sbf[64];
+  constexpr int kEnt = 128;
+  __shared__ int e_start[kEnt + 1];
+  __shared__ int e_meta[kEnt][3];  // charge offset, first point, flags
+  __shared__ float e_shift[kEnt][3];
+  const int4 it = d.items[blockIdx.x];
+  const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
+  const int c0 = blockIdx.y * kMB;
+  const int ncv = min(kMB, d.nch - c0);
+  const int cmax = d.width - c0 - 4;  // last 16-byte chunk start inside the row (width % 4 == 0)
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ng = d.g_n[g];
+  const int nti = min(64, ng - t0);
+  const int TB = nti <= 16 ? 1 : nti <= 32 ? 2 : 4;
+  const int SPL = 4 / TB;
+  const int tb = wave % TB, spart = wave / TB;
+  const bool wave_busy = t0 + tb * 16 < ng;
+  const int tloc = t0 + tb * 16 + (lane & 15);
+  const bool tvalid = tloc < ng;
+  const int tslot = d.g_pt[g] + (tvalid ? tloc : t0);
+  const float* px = static_cast<const float*>(d.px);
+  const float* py = static_cast<const float*>(d.py);
+  const float* pz = static_cast<const float*>(d.pz);
+  const float* qv = static_cast<const float*>(d.q);
+  const float xt = px[tslot], yt = py[tslot], zt = (DIM == 3) ? pz[tslot] : 0.0f;
+  float gxt = 0, gyt = 0, gzt = 0;
+  if (OpT<OP>::geom == 2) {
+    gxt = static_cast<const float*>(d.nx)[tslot];
+    gyt = static_cast<const float*>(d.ny)[tslot];
+    gzt = static_cast<const float*>(d.nz)[tslot];
+  }
+  const int gb = d.g_box[g];
+  const double ctx = d.centers[3 * gb + 0], cty = d.centers[3 * gb + 1], ctz = d.centers[3 * gb + 2];
+  double acc[4][4];
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[nb][r] = 0.0;
+
+  const int nent = se - sb;
+  const bool etab = nent <= kEnt;  // uniform
+  int tot = 0;
+  if (etab) {
+    if (wave == 0) {
+      int n2[2], loc = 0;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = lane * 2 + u;
+        n2[u] = e < nent ? d.e_n[sb + e] : 0;
+        loc += n2[u];
+      }
+      int inc = loc;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+      }
+      int run = inc - loc;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = lane * 2 + u;
+        e_start[e] = run;
+        run += n2[u];
+      }
+      if (lane == 63) e_start[kEnt] = run;
+    }
+    for (int e = tid; e < nent; e += kWG) {
+      const int s = sb + e;
+      e_meta[e][0] = d.e_q[s];
+      e_meta[e][1] = d.e_pt[s];
+      e_meta[e][2] = d.e_flags[s];
+      const int eb = d.e_box[s];
+      e_shift[e][0] = float(ctx - d.centers[3 * eb + 0]);
+      e_shift[e][1] = float(cty - d.centers[3 * eb + 1]);
+      e_shift[e][2] = float(ctz - d.centers[3 * eb + 2]);
+    }
+    __syncthreads();
+    tot = e_start[nent];
+  } else {
+    for (int s = sb; s < se; ++s) tot += d.e_n[s];
+  }
+
+  // row `off` of the tile at stream position tbase -> (entry, local index): through the LDS table
+  // from the tile's first entry, or through the global lists from the tile's cursor
+  int ce = 0, cs = sb, cj = 0;  // tile start: entry (LDS path); list position and offset (global path)
+  auto locate = [&](int tbase, int off, int& s, int& j) {
+    if (etab) {
+      const int pos = tbase + off;
+      int e = ce;
+      while (e_start[e + 1] <= pos) ++e;
+      s = e;
+      j = pos - e_start[e];
+    } else {
+      s = cs;
+      j = cj + off;
+      while (j >= d.e_n[s]) {
+        j -= d.e_n[s];
+        ++s;
+      }
+    }
+  };
+  // move the cursors to the next tile (64 positions on)
+  auto advance = [&](int next) {
+    if (etab) {
+      while (ce < nent && e_start[ce + 1] <= next) ++ce;
+    } else {
+      cj += 64;
+      while (cs < se && cj >= d.e_n[cs]) {
+        cj -= d.e_n[cs];
+        ++cs;
+      }
+    }
+  };
+  // Staging of the tile at stream position tbase (nsrc sources). Every LDS read that locates
+  // rows (entry table walks) happens before the tile's LDS-direct loads are issued: a later LDS
+  // read the compiler cannot tell apart from the loads in flight would wait for them.
+  // Charges: wave w loads 1 KiB chunks 4 w .. 4 w + 3 of the tile, chunk c holding rows
+  // 4 c .. 4 c + 3; per-source data: one thread of four per row, into registers.
+  const int sr = tid >> 2;
+  const bool meta_thread = (tid & 3) == 0;
+  float rp[PS], rsh[3];
+  int rself = -1;
+  auto stage_issue = [&](float* qdst, int tbase, int nsrc) {
+    const float* src[4];
+#pragma unroll
+    for (int ii = 0; ii < 4; ++ii) {
+      const int row = (wave * 4 + ii) * 4 + (lane >> 4);
+      const int rr = row < nsrc ? row : 0;
+      int s, j;
+      locate(tbase, rr, s, j);
+      const int eq = etab ? e_meta[s][0] : d.e_q[s];
+      const int col = min(((lane & 15) ^ ((row & 3) << 2)) << 2, cmax);
+      src[ii] = qv + (long long)(eq + j) * d.width + c0 + col;
+    }
+    int sl = 0, fl = 0;
+    float ox = 0, oy = 0, oz = 0;
+    const bool have = meta_thread && sr < nsrc;
+    if (have) {
+      int s, j;
+      locate(tbase, sr, s, j);
+      if (etab) {
+        sl = e_meta[s][1] + j;
+        fl = e_meta[s][2];
+        ox = e_shift[s][0];
+        oy = e_shift[s][1];
+        oz = e_shift[s][2];
+      } else {
+        sl = d.e_pt[s] + j;
+        fl = d.e_flags[s];
+        const int eb = d.e_box[s];
+        ox = float(ctx - d.centers[3 * eb + 0]);
+        oy = float(cty - d.centers[3 * eb + 1]);
+        oz = float(ctz - d.centers[3 * eb + 2]);
+      }
+      rself = (fl & SKF_SRC_SELF) ? j : -1;
+    } else {
+      rself = -1;
+    }
+#pragma unroll
+    for (int ii = 0; ii < 4; ++ii)
+      __builtin_amdgcn_global_load_lds((global_void_ptr)src[ii], (lds_void_ptr)(qdst + (wave * 4 + ii) * 256), 16, 0, 0);
+    // raw loads only: their values are used after the matrix work (meta_store), so no wait on
+    // them (and on the loads issued before them) is needed here
+    if (meta_thread) {
+#pragma unroll
+      for (int c = 0; c < PS; ++c) rp[c] = 0.0f;
+      if (have) {
+        rp[0] = px[sl];
+        rp[1] = py[sl];
+        if (DIM == 3) rp[2] = pz[sl];
+        if (NG) {
+          rp[3] = static_cast<const float*>(d.nx)[sl];
+          rp[4] = static_cast<const float*>(d.ny)[sl];
+          rp[5] = static_cast<const float*>(d.nz)[sl];
+        }
+        rp[SG] = (fl & SKF_SRC_NEG) ? -1.0f : 1.0f;
+        rsh[0] = ox;
+        rsh[1] = oy;
+        rsh[2] = oz;
+      } else {
+        rp[0] = rp[1] = rp[2] = 1e18f;  // far away with sign 0: contributes exactly zero
+        rsh[0] = rsh[1] = rsh[2] = 0.0f;
+      }
+    }
+  };
+  auto meta_store = [&](float* __restrict__ pdst, int* __restrict__ sdst) {
+    if (!meta_thread) return;
+    pdst[sr * PS + 0] = rp[0] - rsh[0];
+    pdst[sr * PS + 1] = rp[1] - rsh[1];
+    pdst[sr * PS + 2] = rp[2] - rsh[2];
+#pragma unroll
+    for (int c = 3; c < PS; ++c) pdst[sr * PS + c] = rp[c];
+    sdst[sr] = rself;
+  };
+  // matrix work of one staged tile
+  // (restrict parameters: the reads carry alias scopes, so they need not wait for the loads in flight)
+  auto compute = [&](const float* __restrict__ qcur, const float* __restrict__ pcur, const int* __restrict__ scur,
+                     int nsrc) {
+    if (!wave_busy) return;
+    V D[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb) D[nb] = V{0, 0, 0, 0};
+    const int ksteps = (nsrc + 3) >> 2;
+    const int k = lane >> 4;
+    for (int kk = spart; kk < ksteps; kk += SPL) {
+      const int sj = kk * 4 + k;
+      const float* row = &pcur[sj * PS];
+      const float dx = xt - row[0], dy = yt - row[1], dz = (DIM == 3) ? zt - row[2] : 0.0f;
+      float rn = 0;
+      if (OpT<OP>::geom == 1) rn = dx * row[3] + dy * row[4] + dz * row[5];
+      if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
+      float a = kernel_value<float, OP>(dx, dy, dz, rn, 0.0f).re * row[SG];
+      if (scur[sj] == tloc) a = 0.0f;
+      const float* qrow = qcur + sj * kMB + (lane & 15);
+#pragma unroll
+      for (int nb = 0; nb < 4; ++nb) D[nb] = MF::mac(a, qrow[(nb ^ k) * 16], D[nb]);
+    }
+#pragma unroll
+    for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[nb][r] += double(D[nb][r]);
+  };
+  // one tile: prefetch the next into (qn, pn, sn) while computing (qc, pc, sc)
+  auto step = [&](int base, float* qc, float* pc, int* sc, float* qn, float* pn, int* sn) {
+    const int nsrc = min(64, tot - base);
+    const bool more = base + 64 < tot;
+    int nnext = 0;
+    if (more) {
+      advance(base + 64);
+      nnext = min(64, tot - base - 64);
+      stage_issue(qn, base + 64, nnext);
+    }
+    // keep the staging ahead of the matrix work (the compiler may otherwise sink it below)
+    asm volatile("" ::: "memory");
+    compute(qc, pc, sc, nsrc);
+    if (more) meta_store(pn, sn);
+    wait_vm_loads();
+    __syncthreads();
+  };
+  if (tot > 0) {
+    stage_issue(qa, 0, min(64, tot));
+    meta_store(pa, sa);
+    __syncthreads();
+  }
+  // on every path: the target data loaded above is complete, so the tile loop's only
+  // outstanding loads are the staging of the next tile
+  wait_vm_loads();
+  __syncthreads();
+  for (int base = 0; base < tot; base += 128) {
+    step(base, qa, pa, sa, qb, pb, sbf);
+    if (base + 64 < tot) step(base + 64, qb, pb, sbf, qa, pa, sa);
+  }
+  float* uo = static_cast<float*>(d.u);
+#pragma unroll
+  for (int nb = 0; nb < 4; ++nb) {
+    const int ch = nb * 16 + (lane & 15);
+    if (ch >= ncv) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int tl = t0 + tb * 16 + MF::row(lane, r);
+      if (wave_busy && tl < ng) atomic_acc(&uo[(long long)(d.g_out[g] + tl) * d.width + c0 + ch], float(acc[nb][r]));
+    }
+  }
+}
+
 template <typename R, int OP, bool PREC>
 int launch_p2p_mfma(const P2PDev& d, int nitems, hipStream_t st) {
   dim3 grid(nitems, (d.nch + kMB - 1) / kMB);
+  if constexpr (sizeof(R) == 4 && PREC) {
+    // 16-byte LDS-direct loads of charge rows: width a multiple of 4 floats, aligned base
+    if (d.width % 4 == 0 && (reinterpret_cast<unsigned long long>(d.q) & 15ull) == 0) {
+      hipLaunchKernelGGL((p2p_mfma_dma_kernel<OP>), grid, dim3(kWG), 0, st, d);
+      SKF_LAUNCHED();
+      return SKF_OK;
+    }
+  }
   hipLaunchKernelGGL((p2p_mfma_kernel<R, OP, PREC>), grid, dim3(kWG), 0, st, d);
   SKF_LAUNCHED();
   return SKF_OK;

@@ -251,9 +251,12 @@ def test_batched_instances_match_separate_applies():
         assert rel_l2(part.cpu().numpy(), ref) <= FP64_TOL
 
 
-@pytest.mark.parametrize("name,nrhs", [("laplace_2d", 64), ("laplace_3d", 20), ("laplace_dl", 16)])
+@pytest.mark.parametrize("name,nrhs", [("laplace_2d", 64), ("laplace_3d", 20), ("laplace_dl", 16), ("laplace_2d", 18),
+                                       ("laplace_3d", 100), ("laplace_adl", 16)])
 def test_mfma_multi_rhs_precision_form(name, nrhs):
-    """FP32 matrix-core near field (R >= 16) against the FP64 apply on the same build."""
+    """FP32 matrix-core near field (R >= 16) against the FP64 apply on the same build. Widths
+    that are multiples of 4 stage the charge tiles with LDS-direct loads (20 and 100: a partial
+    64-channel block; 100: two blocks); 18 takes the register-staged kernel."""
     from skelfmm_amd.batched import SkelFMMBatched
 
     rng = np.random.default_rng(1)
@@ -267,7 +270,8 @@ def test_mfma_multi_rhs_precision_form(name, nrhs):
         X = rng.random((12000, 3))
         g = rng.normal(size=(12000, 3))
         g /= np.linalg.norm(g, axis=1, keepdims=True)
-        op, leaf = ops.LAPLACE_DOUBLE_LAYER_3D, 150
+        op = ops.LAPLACE_DOUBLE_LAYER_3D if name == "laplace_dl" else ops.LAPLACE_ADJOINT_DOUBLE_LAYER_3D
+        leaf = 150
     fmm = make_build(X, op, tol=1e-6, leaf_size=leaf, geometry=g)
     Q = rng.normal(size=(X.shape[0], nrhs))
     d64 = SkelFMMBatched(fmm.tree, op).build_from_copy(fmm)
@@ -283,7 +287,7 @@ def test_mfma_multi_rhs_precision_form(name, nrhs):
     e_mfma = rel_l2(y_mfma, y64)
     e_valu = rel_l2(y_valu, y64)
     print(f"{name} R={nrhs}: MFMA {e_mfma:.2e} VALU {e_valu:.2e}", flush=True)
-    bound = 2e-5 if name == "laplace_dl" else 2e-6  # 1/r^2 double-layer sums cancel more than 1/r sums
+    bound = 2e-5 if name in ("laplace_dl", "laplace_adl") else 2e-6  # 1/r^2 double-layer sums cancel more
     assert e_mfma < bound and e_valu < bound
     assert rel_l2(y_mfma, y_valu) < 1e-6
     assert rel_l2(y64, BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(Q).numpy()) <= FP64_TOL
