@@ -1551,74 +1551,79 @@ __global__ __launch_bounds__(256) void down_gemm_kernel(const R* __restrict__ T,
   }
 }
 
-template <typename R, bool CX>
-struct Vrows {
-  static constexpr int n = (sizeof(R) * (CX ? 2 : 1) == 16) ? 32 : 64;  // rows per LDS chunk (<= 32 KiB)
-};
-
-// FP32 upward pass with many right-hand sides: the workgroup owns one item (a box's 16 rows of
-// T) and a block of 64 channels (grid.y); the vector rows i it contracts over are staged in LDS
-// in chunks of Vrows rows, once per workgroup, and the four waves (4 rows each) read them from
-// there. Measured faster than up_gemm_kernel in FP32 only (32 Laplace 2D annuli, 3.5M points, 64
-// RHS: 1.54 vs 1.97 ms per apply); FP64 and complex vectors keep up_gemm_kernel (FP64: 2.65 vs
-// 2.35 ms).
-template <typename R, bool CX>
-__global__ __launch_bounds__(256) void up_gemm_lds_kernel(const R* __restrict__ T, const int* __restrict__ box_off,
-                                                      const int* __restrict__ box_bs, const int* __restrict__ box_rank,
-                                                      const long long* __restrict__ box_toff,
-                                                      const int2* __restrict__ items, const R* __restrict__ qvec,
-                                                      R* __restrict__ qskel, int w) {
-  constexpr int CWd = CX ? 2 : 1;
-  constexpr int VR = Vrows<R, CX>::n;
-  __shared__ R xs[VR * 64 * CWd];
+// FP32 interpolation passes with many right-hand sides (real vectors, >= 16 channels) on the
+// matrix cores: v_mfma_f32_16x16x4f32 over 16 rows of T (upward: T[j][i], downward: T^T) and 16
+// channels; the four waves of a workgroup take the four 16-channel blocks of a 64-channel block
+// (grid.y). A and B operands are loaded straight from T and the vectors (four rows of 4 bytes a
+// lane per step, unrolled so that several steps' loads are in flight), accumulated in FP32 like
+// the VALU kernels. Accumulator r of a lane holds D[4 (lane / 16) + r][lane % 16].
+template <bool UP>
+__global__ __launch_bounds__(256) void interp_mfma_kernel(const float* __restrict__ T, const int* __restrict__ box_off,
+                                                          const int* __restrict__ box_bs,
+                                                          const int* __restrict__ box_rank,
+                                                          const long long* __restrict__ box_toff,
+                                                          const int2* __restrict__ items, const float* __restrict__ src,
+                                                          float* __restrict__ dst, int w) {
   const int2 it = items[blockIdx.x];
   const int b = it.x;
   const int off = box_off[b], bs = box_bs[b], r = box_rank[b];
   const long long toff = box_toff[b];
   const int nred = bs - r;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int j0 = it.y + wave * kRowsPerWave;
-  const int nrow = max(0, min(kRowsPerWave, r - j0));  // wave-uniform; the wave still helps stage
-  const int nchan = w / CWd;
-  const int c0 = blockIdx.y * 64;
-  const int c = c0 + lane;
-  const bool cv = c < nchan;
-  R ar[kRowsPerWave], ai[kRowsPerWave];
-#pragma unroll
-  for (int k = 0; k < kRowsPerWave; ++k) ar[k] = ai[k] = 0;
-  for (int i0 = 0; i0 < nred; i0 += VR) {
-    const int ni = min(VR, nred - i0);
-    __syncthreads();
-    for (int e = tid; e < ni * 64; e += 256) {
-      const int ii = e >> 6, cc = c0 + (e & 63);
-      const R* x = qvec + (long long)(off + r + i0 + ii) * w + (long long)cc * CWd;
-#pragma unroll
-      for (int q = 0; q < CWd; ++q) xs[e * CWd + q] = cc < nchan ? x[q] : R(0);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int cb = blockIdx.y * 64 + wave * 16;  // first channel of the wave's block
+  if (cb >= w) return;
+  const int m = lane & 15, k = lane >> 4;
+  const int ch = cb + m;
+  const bool cv = ch < w;
+  const float* Tb = T + toff;
+  if (UP) {
+    // q_skel[j] = q[j] + sum_i T[j][i] q[r + i] for the item's rows j0 .. j0 + 15
+    const int j0 = it.y;
+    const bool rv = j0 + m < r;
+    const float* ta = Tb + (long long)(rv ? j0 + m : j0) * nred;
+    const float* xb = src + (long long)(off + r) * w + (cv ? ch : cb);
+    v4f D = v4f{0, 0, 0, 0};
+#pragma unroll 4
+    for (int i0 = 0; i0 < nred; i0 += 4) {
+      const int i = i0 + k;
+      const bool iv = i < nred;
+      const float a = (rv && iv) ? ta[i] : 0.0f;
+      const float x = (iv && cv) ? xb[(long long)i * w] : 0.0f;
+      D = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x, D, 0, 0, 0);
     }
-    __syncthreads();
-    if (nrow > 0) {
-      const R* tp0 = T + (toff + (long long)j0 * nred + i0) * CWd;
-      for (int ii = 0; ii < ni; ++ii) {
-        const R xr = xs[(ii * 64 + lane) * CWd];
-        const R xi = CX ? xs[(ii * 64 + lane) * CWd + CWd - 1] : R(0);
+    if (cv) {
 #pragma unroll
-        for (int k = 0; k < kRowsPerWave; ++k) {
-          if (k < nrow) {
-            const R* tp = tp0 + ((long long)k * nred + ii) * CWd;
-            cmac<R, CX>(ar[k], ai[k], tp[0], CX ? tp[CWd - 1] : R(0), xr, xi);
-          }
+      for (int q = 0; q < 4; ++q) {
+        const int j = j0 + 4 * k + q;
+        if (j < r) {
+          const long long o = (long long)(off + j) * w + ch;
+          dst[o] = src[o] + D[q];
         }
       }
     }
-  }
-  if (cv) {
+  } else {
+    // u[r + i] += sum_j T[j][i] u_skel[j] for the item's columns, 16 at a time
+    const int ibeg = it.y, iend = min(nred, it.y + kColsPerItem);
+    const float* xb = src + (long long)off * w + (cv ? ch : cb);
+    for (int ib = ibeg; ib < iend; ib += 16) {
+      const int ic = ib + m;
+      const bool iv = ic < iend;
+      v4f D = v4f{0, 0, 0, 0};
+#pragma unroll 4
+      for (int j0 = 0; j0 < r; j0 += 4) {
+        const int j = j0 + k;
+        const bool jv = j < r;
+        const float a = (iv && jv) ? Tb[(long long)j * nred + ic] : 0.0f;
+        const float x = (jv && cv) ? xb[(long long)j * w] : 0.0f;
+        D = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x, D, 0, 0, 0);
+      }
+      if (cv) {
 #pragma unroll
-    for (int k = 0; k < kRowsPerWave; ++k) {
-      if (k < nrow) {
-        const long long o = (long long)(off + j0 + k) * w + (long long)c * CWd;
-        qskel[o] = qvec[o] + ar[k];
-        if (CX) qskel[o + 1] = qvec[o + 1] + ai[k];
+        for (int q = 0; q < 4; ++q) {
+          const int i = ib + 4 * k + q;
+          if (i < iend) dst[(long long)(off + r + i) * w + ch] += D[q];
+        }
       }
     }
   }
@@ -1637,10 +1642,15 @@ int run_up_level(const skf_level_args* L, const void* T, int cx, void* qvec, voi
     const int nchan = cx ? w / 2 : w;
     if (nchan >= 16) {
       const dim3 g2(L->nitems, (nchan + 63) / 64);
-      if (sizeof(R) == 4 && !cx)
-        hipLaunchKernelGGL((up_gemm_lds_kernel<R, false>), g2, dim3(256), 0, st, static_cast<const R*>(T),
-                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
-      else if (cx)
+      if constexpr (sizeof(R) == 4) {
+        if (!cx) {
+          hipLaunchKernelGGL((interp_mfma_kernel<true>), g2, dim3(256), 0, st, static_cast<const float*>(T), L->box_off,
+                             L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
+          SKF_LAUNCHED();
+          return SKF_OK;
+        }
+      }
+      if (cx)
         hipLaunchKernelGGL((up_gemm_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
                            L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
       else
@@ -1676,6 +1686,15 @@ int run_down_level(const skf_level_args* L, const void* T, int cx, void* uvec, v
     const int2* items = reinterpret_cast<const int2*>(L->items);
     const int nchan = cx ? w / 2 : w;
     if (nchan >= 16) {
+      if constexpr (sizeof(R) == 4) {
+        if (!cx) {
+          hipLaunchKernelGGL((interp_mfma_kernel<false>), dim3(L->nitems, (nchan + 63) / 64), dim3(256), 0, st,
+                             static_cast<const float*>(T), L->box_off, L->box_bs, L->box_rank, L->box_toff, items, us,
+                             uv, w);
+          SKF_LAUNCHED();
+          return SKF_OK;
+        }
+      }
       if (cx)
         hipLaunchKernelGGL((down_gemm_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
                            L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
