@@ -1554,9 +1554,11 @@ __global__ __launch_bounds__(256) void down_gemm_kernel(const R* __restrict__ T,
 // FP32 interpolation passes with many right-hand sides (real vectors, >= 16 channels) on the
 // matrix cores: v_mfma_f32_16x16x4f32 over 16 rows of T (upward: T[j][i], downward: T^T) and 16
 // channels; the four waves of a workgroup take the four 16-channel blocks of a 64-channel block
-// (grid.y). A and B operands are loaded straight from T and the vectors (four rows of 4 bytes a
-// lane per step, unrolled so that several steps' loads are in flight), accumulated in FP32 like
+// (grid.y). A and B operands are loaded straight from T and the vectors (4 bytes a lane per
+// operand and step; the loads of kInterpSteps steps are issued before their matrix ops, so they
+// are in flight together), accumulated in FP32 like
 // the VALU kernels. Accumulator r of a lane holds D[4 (lane / 16) + r][lane % 16].
+constexpr int kInterpSteps = 8;  // matrix steps whose operand loads are issued together
 template <bool UP>
 __global__ __launch_bounds__(256) void interp_mfma_kernel(const float* __restrict__ T, const int* __restrict__ box_off,
                                                           const int* __restrict__ box_bs,
@@ -1584,13 +1586,17 @@ __global__ __launch_bounds__(256) void interp_mfma_kernel(const float* __restric
     const float* ta = Tb + (long long)(rv ? j0 + m : j0) * nred;
     const float* xb = src + (long long)(off + r) * w + (cv ? ch : cb);
     v4f D = v4f{0, 0, 0, 0};
-#pragma unroll 4
-    for (int i0 = 0; i0 < nred; i0 += 4) {
-      const int i = i0 + k;
-      const bool iv = i < nred;
-      const float a = (rv && iv) ? ta[i] : 0.0f;
-      const float x = (iv && cv) ? xb[(long long)i * w] : 0.0f;
-      D = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x, D, 0, 0, 0);
+    for (int i0 = 0; i0 < nred; i0 += 4 * kInterpSteps) {
+      float a[kInterpSteps], x[kInterpSteps];
+#pragma unroll
+      for (int u = 0; u < kInterpSteps; ++u) {
+        const int i = i0 + 4 * u + k;
+        const bool iv = i < nred;
+        a[u] = (rv && iv) ? ta[i] : 0.0f;
+        x[u] = (iv && cv) ? xb[(long long)i * w] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < kInterpSteps; ++u) D = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], x[u], D, 0, 0, 0);
     }
     if (cv) {
 #pragma unroll
@@ -1610,13 +1616,17 @@ __global__ __launch_bounds__(256) void interp_mfma_kernel(const float* __restric
       const int ic = ib + m;
       const bool iv = ic < iend;
       v4f D = v4f{0, 0, 0, 0};
-#pragma unroll 4
-      for (int j0 = 0; j0 < r; j0 += 4) {
-        const int j = j0 + k;
-        const bool jv = j < r;
-        const float a = (iv && jv) ? Tb[(long long)j * nred + ic] : 0.0f;
-        const float x = (jv && cv) ? xb[(long long)j * w] : 0.0f;
-        D = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x, D, 0, 0, 0);
+      for (int j0 = 0; j0 < r; j0 += 4 * kInterpSteps) {
+        float a[kInterpSteps], x[kInterpSteps];
+#pragma unroll
+        for (int u = 0; u < kInterpSteps; ++u) {
+          const int j = j0 + 4 * u + k;
+          const bool jv = j < r;
+          a[u] = (iv && jv) ? Tb[(long long)j * nred + ic] : 0.0f;
+          x[u] = (jv && cv) ? xb[(long long)j * w] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < kInterpSteps; ++u) D = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], x[u], D, 0, 0, 0);
       }
       if (cv) {
 #pragma unroll
