@@ -178,12 +178,13 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
               part[k] += kv.re * row[S::dim];
               r0 += kv.re * qt[k];
             } else {
+              // complex multiply-accumulates as two FMAs each (no separate product and add)
               const R sr = row[S::dim], si = row[S::dim + 1];
-              part[2 * k] += kv.re * sr - kv.im * si;
-              part[2 * k + 1] += kv.re * si + kv.im * sr;
+              part[2 * k] = fma(kv.re, sr, fma(-kv.im, si, part[2 * k]));
+              part[2 * k + 1] = fma(kv.re, si, fma(kv.im, sr, part[2 * k + 1]));
               const R tr = qt[2 * k], ti = qt[2 * k + 1];
-              r0 += kv.re * tr - kv.im * ti;
-              r1 += kv.re * ti + kv.im * tr;
+              r0 = fma(kv.re, tr, fma(-kv.im, ti, r0));
+              r1 = fma(kv.re, ti, fma(kv.im, tr, r1));
             }
           }
           rvl[((wave * SUB + jj) * CW + 0) * 64 + lane] = r0;

@@ -132,6 +132,34 @@ def test_precision_form_against_direct_sum():
     assert rel_l2(y32, y64) < 2e-6
 
 
+def test_fp32_helmholtz_fast_trig_accuracy():
+    """The FP32 form's Helmholtz near field with the branch-free reduction and the hardware
+    sine/cosine (taken when kappa times the cloud diameter <= 65536) against the same build with
+    the library's sincosf (max_distance 0 turns the fast path off) and the FP64 apply, at kappa
+    40 (kernel arguments up to ~50 in the near field)."""
+    from skelfmm_amd.batched import SkelFMMBatched
+
+    X = np.random.default_rng(5).random((8000, 3))
+    fmm = make_build(X, ops.HELMHOLTZ_3D, kappa=40.0, tol=1e-6, leaf_size=100)
+    rs = np.random.RandomState(4)
+    q = rs.randn(X.shape[0]) + 1j * rs.randn(X.shape[0])
+    x = torch.as_tensor(q).to(_gpu())
+    y64 = SkelFMMBatched(fmm.tree, fmm.operator, kappa=fmm.kappa).build_from_copy(fmm).matvec(x).cpu().numpy()
+    d32 = SkelFMMBatched(fmm.tree, fmm.operator, kappa=fmm.kappa).build_from_copy(fmm, use_single_precision=True)
+    assert 0 < 40.0 * d32.max_distance <= 65536
+    x32 = torch.as_tensor(q.astype(np.complex64)).to(_gpu())
+    y_fast = d32.matvec(x32).cpu().numpy().astype(np.complex128)
+    d32.max_distance = 0.0
+    for k in list(d32._plans):
+        d32._destroy_plan(k)
+    d32._graphs = {}
+    y_lib = d32.matvec(x32).cpu().numpy().astype(np.complex128)
+    e_fast, e_lib = rel_l2(y_fast, y64), rel_l2(y_lib, y64)
+    print(f"helmholtz kappa 40 FP32: fast trig {e_fast:.2e}, sincosf {e_lib:.2e}", flush=True)
+    assert e_fast <= 1.5 * e_lib + 1e-7
+    assert e_fast < 2e-6
+
+
 def test_precision_form_helmholtz_complex64():
     """Mirrors test_batched.py:469-495: complex64 result within 5e-4 of FP64."""
     rng = np.random.default_rng(3)

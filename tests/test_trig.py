@@ -1,11 +1,13 @@
 """The FP32 form's reduced-range sine/cosine (``sincos_reduced`` in ``csrc/p2p_core.h``).
 
-The constants are read from the header itself and the algorithm (Cody-Waite reduction by
-pi/2 in three FP32 parts, Taylor polynomials on |y| <= pi/4, quadrant selection) is run in
-numpy FP32 with each FMA rounded once; the result must match sin and cos in FP64 to a few
-FP32 ulps over the whole range the host allows (|x| <= 65536), so a wrong digit, a dropped
-term or a swapped quadrant in the header fails here. The device instances are checked end
-to end by the FP32 Helmholtz tests on the GPU.
+The constants are read from the header itself and the reduction (Cody-Waite by 2 pi in three
+FP32 parts, then y / (2 pi) revolutions for the hardware sine and cosine) is run in numpy FP32
+with each FMA rounded once: the revolutions handed to the hardware must equal the exact
+x / (2 pi) minus an integer to a few FP32 ulps of 1/2 over the whole range the host allows
+(|x| <= 65536), so a wrong digit, a dropped part of the split or a wrong scale fails here. The
+hardware sine/cosine itself (v_sin_f32 / v_cos_f32, measured max absolute error 1.3e-7 over
+[-1/2, 1/2] revolutions against FP64 on an MI355X) is checked end to end on the GPU
+(``test_gpu_parity.py::test_fp32_helmholtz_fast_trig_accuracy``).
 """
 
 import os
@@ -33,59 +35,53 @@ def _fma(a, b, c):
     return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
 
 
-def _sincos(x, k):
+def _revolutions(x, k):
+    inv1, c1, c2, c3, inv2 = [np.float32(v) for v in k]
     x = np.float32(x)
-    two_over_pi, c1, c2, c3, s9, s7, s5, s3, c10, c8, c6, c4, c2h, one = [np.float32(v) for v in k]
-    n = np.float32(np.rint(np.float32(x * two_over_pi)))
+    n = np.float32(np.rint(np.float32(x * inv1)))
     y = _fma(-n, c1, x)
     y = _fma(-n, c2, y)
     y = _fma(-n, c3, y)
-    z = np.float32(y * y)
-    ps = _fma(z, s9, s7)
-    ps = _fma(z, ps, s5)
-    ps = _fma(z, ps, s3)
-    sn = _fma(np.float32(y * z), ps, y)
-    pc = _fma(z, c10, c8)
-    pc = _fma(z, pc, c6)
-    pc = _fma(z, pc, c4)
-    pc = _fma(z, pc, c2h)
-    cs = _fma(z, pc, one)
-    q = int(n)
-    sv, cv = (cs, sn) if q & 1 else (sn, cs)
-    return (-sv if q & 2 else sv), (-cv if (q + 1) & 2 else cv)
+    return np.float32(y * inv2), n
 
 
 def test_constants_are_the_mathematical_ones():
     k = _consts()
-    assert len(k) == 14
+    assert len(k) == 5
     getcontext().prec = 50
     pi = Decimal("3.14159265358979323846264338327950288419716939937510")
-    half = pi / 2
-    assert abs(Decimal(float(np.float32(k[0]))) - 2 / pi) < Decimal("1e-7")
-    split = Decimal(float(np.float32(k[1]))) + Decimal(float(np.float32(k[2]))) + Decimal(float(np.float32(k[3])))
-    assert abs(split - half) < Decimal("1e-21"), "pi/2 must be split exactly enough for |n| up to 4e4"
-    taylor = [1 / 362880, -1 / 5040, 1 / 120, -1 / 6, -1 / 3628800, 1 / 40320, -1 / 720, 1 / 24, -0.5, 1.0]
-    np.testing.assert_allclose(k[4:], taylor, rtol=1e-7)
+    two_pi = 2 * pi
+    for inv in (k[0], k[4]):
+        assert abs(Decimal(float(np.float32(inv))) - 1 / two_pi) < Decimal("1e-8")
+    split = sum(Decimal(float(np.float32(c))) for c in k[1:4])
+    assert abs(split - two_pi) < Decimal("1e-21"), "2 pi must be split exactly enough for |n| up to 1.1e4"
+    assert float(np.float32(k[1])) == float(np.float32(2 * np.pi))
 
 
-def test_reduced_sincos_matches_fp64_over_the_allowed_range():
+def test_reduction_matches_exact_revolutions_over_the_allowed_range():
     k = _consts()
     rng = np.random.default_rng(0)
-    xs = np.concatenate([rng.uniform(-4.0, 4.0, 3000), rng.uniform(-65536.0, 65536.0, 3000),
+    xs = np.concatenate([rng.uniform(-8.0, 8.0, 3000), rng.uniform(-65536.0, 65536.0, 3000),
                          np.arange(-40, 41) * (np.pi / 4), [0.0, 1e-30, 65536.0, -65536.0]]).astype(np.float32)
-    err = 0.0
+    getcontext().prec = 40
+    two_pi = 2 * Decimal("3.14159265358979323846264338327950288419716939937510")
+    worst = 0.0
     for x in xs:
-        s, c = _sincos(x, k)
-        xd = float(x)  # the FP32 input is exact in FP64
-        err = max(err, abs(float(s) - np.sin(xd)), abs(float(c) - np.cos(xd)))
-    # reduction rounding grows with |n| as ulp(x) * 2^-24 * |n|-ish; a few FP32 ulps of 1
-    assert err < 4e-7, err
+        t, n = _revolutions(x, k)
+        exact = Decimal(float(x)) / two_pi - Decimal(float(n))  # the FP32 input is exact
+        worst = max(worst, abs(float(Decimal(float(t)) - exact)))
+        # n = rint(fl(x / 2 pi)) may pick the neighbour near a half turn: |t| <= 1/2 + |x| 2^-24 / (2 pi)
+        assert abs(float(t)) <= 0.5 + 1e-3
+        # with an ideal sine of the revolutions the value is within the reduction error
+        assert abs(np.sin(2 * np.pi * float(t)) - np.sin(float(x))) < 4e-7
+    assert worst < 6e-8, worst
 
 
 def test_quadrants_and_signs():
     k = _consts()
     for q in range(-8, 9):
         x = np.float32(q * np.pi / 2 + 0.3)
-        s, c = _sincos(x, k)
+        t, _ = _revolutions(x, k)
         xd = float(x)
-        assert abs(float(s) - np.sin(xd)) < 3e-7 and abs(float(c) - np.cos(xd)) < 3e-7
+        assert abs(np.sin(2 * np.pi * float(t)) - np.sin(xd)) < 3e-7
+        assert abs(np.cos(2 * np.pi * float(t)) - np.cos(xd)) < 3e-7
