@@ -142,7 +142,8 @@ typedef struct skf_p2p_args {
      skf_jit_compile) for channel blocks of 1, 2, 4 and 8, in this argument's precision */
   void* user_fn[4];
   /* an upper bound of every pair distance (e.g. the point cloud's diameter), 0 = unknown; with
-     it the FP32 form evaluates e^{i kappa r} with a branch-free reduced sine/cosine when
+     it the 3D Helmholtz kernels evaluate e^{i kappa r} with a branch-free reduced sine/cosine
+     (FP32: hardware sine/cosine after the reduction, FP64: polynomials) when
      kappa * max_distance <= 65536 */
   double max_distance;
 } skf_p2p_args;

@@ -77,6 +77,38 @@ __device__ inline void sincos_reduced(float x, float* sp, float* cp) {
   *cp = __builtin_amdgcn_cosf(t);
 }
 
+// The same for FP64 (the plain form), from the mathematics alone: Cody-Waite reduction
+// x = n (pi/2) + y with pi/2 split in two FP64 parts (each step one FMA, so one rounding;
+// the omitted third part is below 1e-28 for |n| < 2^16), Taylor polynomials of sin (to y^15)
+// and cos (to y^16) on |y| <= pi/4 (truncation below 5e-17), and the quadrant of n: about
+// 2e-16 absolute, no branches.
+__device__ inline void sincos_reduced(double x, double* sp, double* cp) {
+  const double n = rint(x * 0.6366197723675814);
+  double y = fma(-n, 1.5707963267948966, x);
+  y = fma(-n, 6.123233995736766e-17, y);
+  const double z = y * y;
+  double ps = fma(z, -7.647163731819816e-13, 1.6059043836821613e-10);
+  ps = fma(z, ps, -2.505210838544172e-08);
+  ps = fma(z, ps, 2.7557319223985893e-06);
+  ps = fma(z, ps, -0.0001984126984126984);
+  ps = fma(z, ps, 0.008333333333333333);
+  ps = fma(z, ps, -0.16666666666666666);
+  const double sn = fma(y * z, ps, y);
+  double pc = fma(z, 4.779477332387385e-14, -1.1470745597729725e-11);
+  pc = fma(z, pc, 2.08767569878681e-09);
+  pc = fma(z, pc, -2.755731922398589e-07);
+  pc = fma(z, pc, 2.48015873015873e-05);
+  pc = fma(z, pc, -0.001388888888888889);
+  pc = fma(z, pc, 0.041666666666666664);
+  pc = fma(z, pc, -0.5);
+  const double cs = fma(z, pc, 1.0);
+  const int q = int(n);
+  const double sv = (q & 1) ? cs : sn;
+  const double cv = (q & 1) ? sn : cs;
+  *sp = (q & 2) ? -sv : sv;
+  *cp = ((q + 1) & 2) ? -cv : cv;
+}
+
 template <>
 struct Mth<float> {
   static constexpr bool fp32_bias = true;
@@ -104,21 +136,20 @@ template <> struct OpT<SKF_HELMHOLTZ_DL_3D> { static constexpr int dim = 3, cplx
 template <> struct OpT<SKF_HELMHOLTZ_ADL_3D> { static constexpr int dim = 3, cplx = 1, geom = 2; };
 
 // K(x, y) for dx = x - y and the normal dot product rn = dx . n (source or target normal).
-// FT: the FP32 form's Helmholtz arguments are bounded by kTrigFastMax (the host checks it and
-// selects these instances)
+// FT: the Helmholtz arguments are bounded by kTrigFastMax (the host checks it and selects these
+// instances)
 template <typename R, bool FT>
 __device__ inline void sincos_k(R x, R* s, R* c) {
-  if constexpr (FT && sizeof(R) == 4)
+  if constexpr (FT)
     sincos_reduced(x, s, c);
   else
     Mth<R>::sc(x, s, c);
 }
 
-// instances with the reduced sincos exist for the FP32 3D Helmholtz kernels only
+// instances with the reduced sincos exist for the 3D Helmholtz kernels (both precisions)
 template <typename R, int OP>
 struct HasFastTrig {
-  static constexpr bool value = sizeof(R) == 4 && (OP == SKF_HELMHOLTZ_3D || OP == SKF_HELMHOLTZ_DL_3D ||
-                                                   OP == SKF_HELMHOLTZ_ADL_3D);
+  static constexpr bool value = OP == SKF_HELMHOLTZ_3D || OP == SKF_HELMHOLTZ_DL_3D || OP == SKF_HELMHOLTZ_ADL_3D;
 };
 
 template <typename R, int OP, bool FT = false>
@@ -204,7 +235,7 @@ struct P2PDev {
   int nch;  // channels (complex channels for complex kernels)
   double kappa;
   double* urev;  // reverse outputs of the symmetric kernel (FP64; u itself in the FP64 form)
-  int fast_trig;  // FP32 form: every kernel argument kappa r is within kTrigFastMax
+  int fast_trig;  // every kernel argument kappa r is within kTrigFastMax
 };
 
 __device__ inline void atomic_acc(double* p, double v) { unsafeAtomicAdd(p, v); }

@@ -85,3 +85,73 @@ def test_quadrants_and_signs():
         xd = float(x)
         assert abs(np.sin(2 * np.pi * float(t)) - np.sin(xd)) < 3e-7
         assert abs(np.cos(2 * np.pi * float(t)) - np.cos(xd)) < 3e-7
+
+
+# ------------------------------------------------------------------ FP64 (the plain form)
+def _body64():
+    src = open(HEADER, encoding="utf-8").read()
+    a = src.index("__device__ inline void sincos_reduced(double x")
+    b = src.index("\n}\n", a)
+    return src[a:b]
+
+
+def _consts64():
+    return [float(m) for m in re.findall(r"(?<![\w.])(-?\d+\.\d+(?:e[-+]?\d+)?)(?![\w.])", _body64())]
+
+
+def _fma64(a, b, c):
+    from fractions import Fraction
+
+    return float(Fraction(a) * Fraction(b) + Fraction(c))  # exact, then one rounding
+
+
+def _sincos64(x, k):
+    two_over_pi, c1, c2 = k[:3]
+    sin_c = k[3:10]   # y^15 .. y^3 coefficients
+    cos_c = k[10:17]  # y^16 .. y^4
+    c2h, one = k[17], k[18]
+    n = float(round(x * two_over_pi))  # rint: round half to even
+    y = _fma64(-n, c1, x)
+    y = _fma64(-n, c2, y)
+    z = y * y
+    ps = _fma64(z, sin_c[0], sin_c[1])
+    for c in sin_c[2:]:
+        ps = _fma64(z, ps, c)
+    sn = _fma64(y * z, ps, y)
+    pc = _fma64(z, cos_c[0], cos_c[1])
+    for c in cos_c[2:] + [c2h]:
+        pc = _fma64(z, pc, c)
+    cs = _fma64(z, pc, one)
+    q = int(n)
+    sv, cv = (cs, sn) if q & 1 else (sn, cs)
+    return (-sv if q & 2 else sv), (-cv if (q + 1) & 2 else cv)
+
+
+def test_fp64_constants_are_the_mathematical_ones():
+    import math
+
+    k = _consts64()
+    assert len(k) == 19
+    getcontext().prec = 60
+    pi = Decimal("3.14159265358979323846264338327950288419716939937510582097494459")
+    assert k[0] == float(2 / pi)
+    assert k[1] == float(pi / 2) and k[2] == float(pi / 2 - Decimal(k[1]))
+    sin_ref = [(-1) ** j / Decimal(math.factorial(2 * j + 1)) for j in range(7, 0, -1)]
+    cos_ref = [(-1) ** j / Decimal(math.factorial(2 * j)) for j in range(8, 0, -1)]
+    assert k[3:10] == [float(v) for v in sin_ref]
+    assert k[10:18] == [float(v) for v in cos_ref] and k[18] == 1.0
+
+
+def test_fp64_sincos_matches_libm_over_the_allowed_range():
+    import math
+
+    k = _consts64()
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(-8.0, 8.0, 400), rng.uniform(-65536.0, 65536.0, 400),
+                         np.arange(-20, 21) * (np.pi / 4), [0.0, 1e-300, 65536.0, -65536.0, 1e-8]])
+    err = 0.0
+    for x in xs:
+        s, c = _sincos64(float(x), k)
+        err = max(err, abs(s - math.sin(float(x))), abs(c - math.cos(float(x))))
+    # a few ulps of 1 (reduction ~1.1e-16, polynomial ~1 ulp, libm reference <= 1 ulp)
+    assert err < 5e-16, err
