@@ -553,47 +553,52 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   }
 }
 
-// FP32 form of the matrix-core near field with the charge tiles staged by LDS-direct loads
+// The matrix-core near field with the charge tiles staged by LDS-direct loads
 // (global_load_lds_dwordx4: lane l's 16 bytes land at the instruction's LDS base + 16 l) into
 // two LDS buffers: the loads of tile i + 1 are issued before the matrix work of tile i and
 // waited for after it, so the staging latency overlaps the matrix work without holding the
-// charges in registers. A 64 x 64 charge tile is stored unpadded, the 16-byte chunk q of row r
-// at chunk q ^ 4 (r & 3): the B-operand reads (rows 4 kk .. 4 kk + 3, 16 columns each) fall on
-// distinct banks. Staged charges are raw: the entry sign multiplies the kernel value, and rows
-// past the tile's sources load row 0 with sign 0. The two buffers are distinct arrays and the
-// tile loop is unrolled by two, so the compiler can tell the LDS reads of the current tile from
-// the loads in flight. Needs width % 4 == 0 and 16-byte aligned charges.
+// charges in registers. A tile is TR sources x 64 channels (16 KiB: TR = 64 in FP32, 32 in
+// FP64), stored unpadded with channel c of row r at c ^ 16 (r & 3): the B-operand reads (rows
+// 4 kk .. 4 kk + 3, 16 channels each) fall on distinct banks. Staged charges are raw: the
+// entry sign multiplies the kernel value, and rows past the tile's sources load row 0 with
+// sign 0. The two buffers are distinct arrays and the tile loop is unrolled by two, so the
+// compiler can tell the LDS reads of the current tile from the loads in flight. Needs a width
+// that is a multiple of 16 bytes' worth of scalars and 16-byte aligned charges.
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
 typedef __attribute__((address_space(1))) void* global_void_ptr;
 
 // vmcnt(0), other counters unconstrained (gfx9 s_waitcnt encoding)
 __device__ inline void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-template <int OP>
+template <typename R, int OP, bool PREC>
 __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   static_assert(OpT<OP>::cplx == 0, "real kernels only");
   using S = Stage<OP, 1>;
-  using MF = Mfma<float>;
+  using MF = Mfma<R>;
   typedef typename MF::V V;
   constexpr int DIM = S::dim;
   constexpr int NG = S::ng;
   constexpr int PS = NG ? 8 : 4;  // per-source row: x, y, z, sign (+ normals)
   constexpr int SG = NG ? 6 : 3;  // position of the sign in the row
-  __shared__ __attribute__((aligned(16))) float qa[64 * kMB];
-  __shared__ __attribute__((aligned(16))) float qb[64 * kMB];
-  __shared__ float pa[64 * PS];
-  __shared__ float pb[64 * PS];
-  __shared__ int sa[64];
-  __shared__ int sbf[64];
+  constexpr int TR = sizeof(R) == 4 ? 64 : 32;         // sources per tile
+  constexpr int EPC = 16 / int(sizeof(R));             // scalars per 16-byte chunk
+  constexpr int CPR = kMB / EPC;                       // chunks per tile row
+  constexpr int QSH = sizeof(R) == 4 ? 2 : 3;          // chunk-index shift of the row swizzle
+  __shared__ __attribute__((aligned(16))) R qa[TR * kMB];
+  __shared__ __attribute__((aligned(16))) R qb[TR * kMB];
+  __shared__ R pa[TR * PS];
+  __shared__ R pb[TR * PS];
+  __shared__ int sa[TR];
+  __shared__ int sbf[TR];
   constexpr int kEnt = 128;
   __shared__ int e_start[kEnt + 1];
   __shared__ int e_meta[kEnt][3];  // charge offset, first point, flags
-  __shared__ float e_shift[kEnt][3];
+  __shared__ R e_shift[kEnt][3];
   const int4 it = d.items[blockIdx.x];
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
   const int c0 = blockIdx.y * kMB;
   const int ncv = min(kMB, d.nch - c0);
-  const int cmax = d.width - c0 - 4;  // last 16-byte chunk start inside the row (width % 4 == 0)
+  const int cmax = d.width - c0 - EPC;  // last 16-byte chunk start inside the row (width % EPC == 0)
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ng = d.g_n[g];
@@ -605,24 +610,38 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   const int tloc = t0 + tb * 16 + (lane & 15);
   const bool tvalid = tloc < ng;
   const int tslot = d.g_pt[g] + (tvalid ? tloc : t0);
-  const float* px = static_cast<const float*>(d.px);
-  const float* py = static_cast<const float*>(d.py);
-  const float* pz = static_cast<const float*>(d.pz);
-  const float* qv = static_cast<const float*>(d.q);
-  const float xt = px[tslot], yt = py[tslot], zt = (DIM == 3) ? pz[tslot] : 0.0f;
-  float gxt = 0, gyt = 0, gzt = 0;
+  const R* px = static_cast<const R*>(d.px);
+  const R* py = static_cast<const R*>(d.py);
+  const R* pz = static_cast<const R*>(d.pz);
+  const R* qv = static_cast<const R*>(d.q);
+  const R xt = px[tslot], yt = py[tslot], zt = (DIM == 3) ? pz[tslot] : R(0);
+  R gxt = 0, gyt = 0, gzt = 0;
   if (OpT<OP>::geom == 2) {
-    gxt = static_cast<const float*>(d.nx)[tslot];
-    gyt = static_cast<const float*>(d.ny)[tslot];
-    gzt = static_cast<const float*>(d.nz)[tslot];
+    gxt = static_cast<const R*>(d.nx)[tslot];
+    gyt = static_cast<const R*>(d.ny)[tslot];
+    gzt = static_cast<const R*>(d.nz)[tslot];
   }
-  const int gb = d.g_box[g];
-  const double ctx = d.centers[3 * gb + 0], cty = d.centers[3 * gb + 1], ctz = d.centers[3 * gb + 2];
-  double acc[4][4];
+  // precision form: sources are shifted into the target box's frame (the FP64 form has no centres)
+  double ctx = 0, cty = 0, ctz = 0;
+  if (PREC) {
+    const int gb = d.g_box[g];
+    ctx = d.centers[3 * gb + 0];
+    cty = d.centers[3 * gb + 1];
+    ctz = d.centers[3 * gb + 2];
+  }
+  // FP32: matrix sums per tile, added into FP64 accumulators after every tile; FP64: the matrix
+  // accumulators themselves run across all tiles
+  constexpr bool FOLD = sizeof(R) == 4;
+  double acc[FOLD ? 4 : 1][4];
+  V D[4];
 #pragma unroll
-  for (int nb = 0; nb < 4; ++nb)
+  for (int nb = 0; nb < 4; ++nb) D[nb] = V{0, 0, 0, 0};
+  if constexpr (FOLD) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) acc[nb][r] = 0.0;
+    for (int nb = 0; nb < 4; ++nb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[nb][r] = 0.0;
+  }
 
   const int nent = se - sb;
   const bool etab = nent <= kEnt;  // uniform
@@ -656,10 +675,16 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
       e_meta[e][0] = d.e_q[s];
       e_meta[e][1] = d.e_pt[s];
       e_meta[e][2] = d.e_flags[s];
-      const int eb = d.e_box[s];
-      e_shift[e][0] = float(ctx - d.centers[3 * eb + 0]);
-      e_shift[e][1] = float(cty - d.centers[3 * eb + 1]);
-      e_shift[e][2] = float(ctz - d.centers[3 * eb + 2]);
+      R ox = 0, oy = 0, oz = 0;
+      if (PREC) {
+        const int eb = d.e_box[s];
+        ox = R(ctx - d.centers[3 * eb + 0]);
+        oy = R(cty - d.centers[3 * eb + 1]);
+        oz = R(ctz - d.centers[3 * eb + 2]);
+      }
+      e_shift[e][0] = ox;
+      e_shift[e][1] = oy;
+      e_shift[e][2] = oz;
     }
     __syncthreads();
     tot = e_start[nent];
@@ -686,12 +711,12 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
       }
     }
   };
-  // move the cursors to the next tile (64 positions on)
+  // move the cursors to the next tile (TR positions on)
   auto advance = [&](int next) {
     if (etab) {
       while (ce < nent && e_start[ce + 1] <= next) ++ce;
     } else {
-      cj += 64;
+      cj += TR;
       while (cs < se && cj >= d.e_n[cs]) {
         cj -= d.e_n[cs];
         ++cs;
@@ -702,25 +727,26 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   // rows (entry table walks) happens before the tile's LDS-direct loads are issued: a later LDS
   // read the compiler cannot tell apart from the loads in flight would wait for them.
   // Charges: wave w loads 1 KiB chunks 4 w .. 4 w + 3 of the tile, chunk c holding rows
-  // 4 c .. 4 c + 3; per-source data: one thread of four per row, into registers.
+  // (64 / CPR) c .. ; per-source data: one thread of four per row, into registers.
   const int sr = tid >> 2;
-  const bool meta_thread = (tid & 3) == 0;
-  float rp[PS], rsh[3];
+  const bool meta_thread = (tid & 3) == 0 && sr < TR;
+  R rp[PS], rsh[3];
   int rself = -1;
-  auto stage_issue = [&](float* qdst, int tbase, int nsrc) {
-    const float* src[4];
+  auto stage_issue = [&](R* qdst, int tbase, int nsrc) {
+    const R* src[4];
 #pragma unroll
     for (int ii = 0; ii < 4; ++ii) {
-      const int row = (wave * 4 + ii) * 4 + (lane >> 4);
+      const int slot = (wave * 4 + ii) * 64 + lane;  // 16-byte chunk of the tile
+      const int row = slot / CPR, q = slot % CPR;
       const int rr = row < nsrc ? row : 0;
       int s, j;
       locate(tbase, rr, s, j);
       const int eq = etab ? e_meta[s][0] : d.e_q[s];
-      const int col = min(((lane & 15) ^ ((row & 3) << 2)) << 2, cmax);
+      const int col = min((q ^ ((row & 3) << QSH)) * EPC, cmax);
       src[ii] = qv + (long long)(eq + j) * d.width + c0 + col;
     }
     int sl = 0, fl = 0;
-    float ox = 0, oy = 0, oz = 0;
+    R ox = 0, oy = 0, oz = 0;
     const bool have = meta_thread && sr < nsrc;
     if (have) {
       int s, j;
@@ -734,10 +760,12 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
       } else {
         sl = d.e_pt[s] + j;
         fl = d.e_flags[s];
-        const int eb = d.e_box[s];
-        ox = float(ctx - d.centers[3 * eb + 0]);
-        oy = float(cty - d.centers[3 * eb + 1]);
-        oz = float(ctz - d.centers[3 * eb + 2]);
+        if (PREC) {
+          const int eb = d.e_box[s];
+          ox = R(ctx - d.centers[3 * eb + 0]);
+          oy = R(cty - d.centers[3 * eb + 1]);
+          oz = R(ctz - d.centers[3 * eb + 2]);
+        }
       }
       rself = (fl & SKF_SRC_SELF) ? j : -1;
     } else {
@@ -745,32 +773,33 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
     }
 #pragma unroll
     for (int ii = 0; ii < 4; ++ii)
-      __builtin_amdgcn_global_load_lds((global_void_ptr)src[ii], (lds_void_ptr)(qdst + (wave * 4 + ii) * 256), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds((global_void_ptr)src[ii], (lds_void_ptr)(qdst + (wave * 4 + ii) * 64 * EPC), 16,
+                                       0, 0);
     // raw loads only: their values are used after the matrix work (meta_store), so no wait on
     // them (and on the loads issued before them) is needed here
     if (meta_thread) {
 #pragma unroll
-      for (int c = 0; c < PS; ++c) rp[c] = 0.0f;
+      for (int c = 0; c < PS; ++c) rp[c] = R(0);
       if (have) {
         rp[0] = px[sl];
         rp[1] = py[sl];
         if (DIM == 3) rp[2] = pz[sl];
         if (NG) {
-          rp[3] = static_cast<const float*>(d.nx)[sl];
-          rp[4] = static_cast<const float*>(d.ny)[sl];
-          rp[5] = static_cast<const float*>(d.nz)[sl];
+          rp[3] = static_cast<const R*>(d.nx)[sl];
+          rp[4] = static_cast<const R*>(d.ny)[sl];
+          rp[5] = static_cast<const R*>(d.nz)[sl];
         }
-        rp[SG] = (fl & SKF_SRC_NEG) ? -1.0f : 1.0f;
+        rp[SG] = (fl & SKF_SRC_NEG) ? R(-1) : R(1);
         rsh[0] = ox;
         rsh[1] = oy;
         rsh[2] = oz;
       } else {
-        rp[0] = rp[1] = rp[2] = 1e18f;  // far away with sign 0: contributes exactly zero
-        rsh[0] = rsh[1] = rsh[2] = 0.0f;
+        rp[0] = rp[1] = rp[2] = R(1e18);  // far away with sign 0: contributes exactly zero
+        rsh[0] = rsh[1] = rsh[2] = R(0);
       }
     }
   };
-  auto meta_store = [&](float* __restrict__ pdst, int* __restrict__ sdst) {
+  auto meta_store = [&](R* __restrict__ pdst, int* __restrict__ sdst) {
     if (!meta_thread) return;
     pdst[sr * PS + 0] = rp[0] - rsh[0];
     pdst[sr * PS + 1] = rp[1] - rsh[1];
@@ -781,41 +810,44 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   };
   // matrix work of one staged tile
   // (restrict parameters: the reads carry alias scopes, so they need not wait for the loads in flight)
-  auto compute = [&](const float* __restrict__ qcur, const float* __restrict__ pcur, const int* __restrict__ scur,
+  auto compute = [&](const R* __restrict__ qcur, const R* __restrict__ pcur, const int* __restrict__ scur,
                      int nsrc) {
     if (!wave_busy) return;
-    V D[4];
+    if constexpr (FOLD) {
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb) D[nb] = V{0, 0, 0, 0};
+      for (int nb = 0; nb < 4; ++nb) D[nb] = V{0, 0, 0, 0};
+    }
     const int ksteps = (nsrc + 3) >> 2;
     const int k = lane >> 4;
     for (int kk = spart; kk < ksteps; kk += SPL) {
       const int sj = kk * 4 + k;
-      const float* row = &pcur[sj * PS];
-      const float dx = xt - row[0], dy = yt - row[1], dz = (DIM == 3) ? zt - row[2] : 0.0f;
-      float rn = 0;
+      const R* row = &pcur[sj * PS];
+      const R dx = xt - row[0], dy = yt - row[1], dz = (DIM == 3) ? zt - row[2] : R(0);
+      R rn = 0;
       if (OpT<OP>::geom == 1) rn = dx * row[3] + dy * row[4] + dz * row[5];
       if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
-      float a = kernel_value<float, OP>(dx, dy, dz, rn, 0.0f).re * row[SG];
-      if (scur[sj] == tloc) a = 0.0f;
-      const float* qrow = qcur + sj * kMB + (lane & 15);
+      R a = kernel_value<R, OP>(dx, dy, dz, rn, R(0)).re * row[SG];
+      if (scur[sj] == tloc) a = R(0);
+      const R* qrow = qcur + sj * kMB + (lane & 15);
 #pragma unroll
       for (int nb = 0; nb < 4; ++nb) D[nb] = MF::mac(a, qrow[(nb ^ k) * 16], D[nb]);
     }
+    if constexpr (FOLD) {
 #pragma unroll
-    for (int nb = 0; nb < 4; ++nb)
+      for (int nb = 0; nb < 4; ++nb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) acc[nb][r] += double(D[nb][r]);
+        for (int r = 0; r < 4; ++r) acc[nb][r] += double(D[nb][r]);
+    }
   };
   // one tile: prefetch the next into (qn, pn, sn) while computing (qc, pc, sc)
-  auto step = [&](int base, float* qc, float* pc, int* sc, float* qn, float* pn, int* sn) {
-    const int nsrc = min(64, tot - base);
-    const bool more = base + 64 < tot;
+  auto step = [&](int base, R* qc, R* pc, int* sc, R* qn, R* pn, int* sn) {
+    const int nsrc = min(TR, tot - base);
+    const bool more = base + TR < tot;
     int nnext = 0;
     if (more) {
-      advance(base + 64);
-      nnext = min(64, tot - base - 64);
-      stage_issue(qn, base + 64, nnext);
+      advance(base + TR);
+      nnext = min(TR, tot - base - TR);
+      stage_issue(qn, base + TR, nnext);
     }
     // keep the staging ahead of the matrix work (the compiler may otherwise sink it below)
     asm volatile("" ::: "memory");
@@ -825,7 +857,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
     __syncthreads();
   };
   if (tot > 0) {
-    stage_issue(qa, 0, min(64, tot));
+    stage_issue(qa, 0, min(TR, tot));
     meta_store(pa, sa);
     __syncthreads();
   }
@@ -833,11 +865,11 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   // outstanding loads are the staging of the next tile
   wait_vm_loads();
   __syncthreads();
-  for (int base = 0; base < tot; base += 128) {
+  for (int base = 0; base < tot; base += 2 * TR) {
     step(base, qa, pa, sa, qb, pb, sbf);
-    if (base + 64 < tot) step(base + 64, qb, pb, sbf, qa, pa, sa);
+    if (base + TR < tot) step(base + TR, qb, pb, sbf, qa, pa, sa);
   }
-  float* uo = static_cast<float*>(d.u);
+  R* uo = static_cast<R*>(d.u);
 #pragma unroll
   for (int nb = 0; nb < 4; ++nb) {
     const int ch = nb * 16 + (lane & 15);
@@ -845,7 +877,8 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int tl = t0 + tb * 16 + MF::row(lane, r);
-      if (wave_busy && tl < ng) atomic_acc(&uo[(long long)(d.g_out[g] + tl) * d.width + c0 + ch], float(acc[nb][r]));
+      if (wave_busy && tl < ng) atomic_acc(&uo[(long long)(d.g_out[g] + tl) * d.width + c0 + ch],
+                                          FOLD ? R(acc[FOLD ? nb : 0][r]) : R(D[nb][r]));
     }
   }
 }
@@ -853,13 +886,11 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
 template <typename R, int OP, bool PREC>
 int launch_p2p_mfma(const P2PDev& d, int nitems, hipStream_t st) {
   dim3 grid(nitems, (d.nch + kMB - 1) / kMB);
-  if constexpr (sizeof(R) == 4 && PREC) {
-    // 16-byte LDS-direct loads of charge rows: width a multiple of 4 floats, aligned base
-    if (d.width % 4 == 0 && (reinterpret_cast<unsigned long long>(d.q) & 15ull) == 0) {
-      hipLaunchKernelGGL((p2p_mfma_dma_kernel<OP>), grid, dim3(kWG), 0, st, d);
-      SKF_LAUNCHED();
-      return SKF_OK;
-    }
+  // 16-byte LDS-direct loads of charge rows: width a multiple of 16 bytes' worth, aligned base
+  if (d.width % (16 / int(sizeof(R))) == 0 && (reinterpret_cast<unsigned long long>(d.q) & 15ull) == 0) {
+    hipLaunchKernelGGL((p2p_mfma_dma_kernel<R, OP, PREC>), grid, dim3(kWG), 0, st, d);
+    SKF_LAUNCHED();
+    return SKF_OK;
   }
   hipLaunchKernelGGL((p2p_mfma_kernel<R, OP, PREC>), grid, dim3(kWG), 0, st, d);
   SKF_LAUNCHED();
