@@ -444,6 +444,8 @@ def main():
     ap.add_argument("--instances", type=int, default=32)
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="skip the near-field autotuner (on by default)")
+    ap.add_argument("--no-gpu-build", action="store_true",
+                    help="laplace3d_cube: skip the second build with the IDs on the GPU")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md, sharding)",
@@ -576,6 +578,29 @@ def main():
     e32 = float(np.linalg.norm(y32h[idx] - ref) / np.linalg.norm(ref))
     log(f"rel-l2 vs direct sum at {idx.size} targets: FP64 {e64:.3e}, FP32 {e32:.3e}")
 
+    # the same tree skeletonized with the IDs on the GPU (batched adaptive pivoted QR), timed and
+    # checked like the host build
+    gpu_build = None
+    if not args.no_gpu_build:
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        fg = RecursiveSkeletonization(tree, op, kappa=0.0, tol=args.tol)
+        fg.skel_tree(npoints_max=500, device="gpu")
+        torch.cuda.synchronize()
+        t_gid = time.perf_counter() - tg
+        tg = time.perf_counter()
+        fg.setup_lists()
+        dg = SkelFMMBatched(tree, op).build_from_copy(fg, use_single_precision=True)
+        torch.cuda.synchronize()
+        t_gset = time.perf_counter() - tg
+        yg = dg.matvec(q32).cpu().numpy().astype(np.float64)
+        eg = float(np.linalg.norm(yg[idx] - ref) / np.linalg.norm(ref))
+        gpu_build = {"id_s": round(t_gid, 3), "lists_upload_s": round(t_gset, 2),
+                     "build_s": round(t_tree + t_gid + t_gset, 2), "rank_sum": int(fg.rank_list.sum()),
+                     "host_rank_sum": int(fmm.rank_list.sum()), "fp32_rel_l2_err": eg}
+        log(f"GPU build: IDs {t_gid:.2f}s, lists+upload {t_gset:.2f}s, FP32 apply error {eg:.3e}")
+        del dg, fg
+
     cpu = None
     if not args.no_cpu:
         ncores = min(16, len(os.sched_getaffinity(0)))
@@ -618,6 +643,7 @@ def main():
         "fp32_over_fp64_speedup": round(t64 / t32, 3),
         "build_s": round(build_s, 2),
         "build_split_s": {"tree": round(t_tree, 2), "skel": round(t_skel, 2), "lists_upload": round(t_setup, 2)},
+        "gpu_build": gpu_build,
         "roofline": _p2p_roofline(d32, d64, s32, s64, st32, st64, p32, p64, tf32, tf64, fpi, N, args),
         "cpu_baseline": cpu,
     }
