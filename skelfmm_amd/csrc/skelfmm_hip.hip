@@ -85,10 +85,23 @@ __device__ inline void wave_sync_lds() {
   asm volatile("" ::: "memory");
 }
 
+// Entry table of a work item (LDS), for items with at most kSymEnt source entries: stream
+// offsets (prefix sums of the entry sizes), first point, charge offset, flags and, in the
+// precision form, the shift into the target box's frame. Items with more entries stage entry
+// by entry instead.
+constexpr int kSymEnt = 128;
+template <typename R>
+struct SymEnt {
+  int start[kSymEnt + 1];
+  int pt[kSymEnt], q[kSymEnt], fl[kSymEnt];
+  R sh[kSymEnt][3];
+};
+
 // One pass of the symmetric kernel over targets tp .. tp + 64 TPL - 1 of the item (TPL per lane).
 template <typename R, int OP, bool PREC, int UNR, int TPL, bool FT>
 __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, R* __restrict__ rvl,
-                                         double* __restrict__ red, int tp, int tend, int t0, int sb, int se,
+                                         double* __restrict__ red, int* __restrict__ sidx, const SymEnt<R>* et,
+                                         bool etab, int nent, int tot, int tp, int tend, int t0, int sb, int se,
                                          double ctx, double cty, double ctz) {
   using S = Stage<OP, 1>;
   constexpr int NV = S::nv;
@@ -122,93 +135,141 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
 #pragma unroll
   for (int a = 0; a < TPL * CW; ++a) acc[a] = 0.0;
 
-  for (int s = sb; s < se; ++s) {
-    const int ps = d.e_pt[s], ns = d.e_n[s], qo = d.e_q[s], fl = d.e_flags[s];
-    const double sgn = (fl & SKF_SRC_NEG) ? -1.0 : 1.0;
-    // precision form: sources are shifted into the target box's frame (minus the difference of
-    // the centres) as they are staged, once per source for all targets of the item
-    R ox = 0, oy = 0, oz = 0;
-    if (PREC) {
-      const int eb = d.e_box[s];
-      ox = R(ctx - d.centers[3 * eb + 0]);
-      oy = R(cty - d.centers[3 * eb + 1]);
-      oz = R(ctz - d.centers[3 * eb + 2]);
-    }
-    for (int base = 0; base < ns; base += kWG) {
-      {
-        const int j = base + tid;
-        R* row = &tile[tid * NV];
-        if (j < ns) {
-          const int sl = ps + j;
-          row[0] = PREC ? px[sl] - ox : px[sl];
-          row[1] = PREC ? py[sl] - oy : py[sl];
-          if (S::dim == 3) row[2] = PREC ? pz[sl] - oz : pz[sl];
-          const R* qs = qv + (long long)(qo + j) * w;
+  // evaluation of one staged tile of nsc sources (4 waves x 64); dst(j, c, v) adds the reverse
+  // sum v of channel c of the tile's source j; returns with the forward partials in part[]
+  R part[TPL * CW];
+  auto tile_eval = [&](int nsc, auto&& dst) {
+    const int wb = wave * 64;  // wave-uniform, within the tile
+    const int cnt = __builtin_amdgcn_readfirstlane(max(0, min(64, nsc - wb)));
+    const R* tw = &tile[wave * 64 * NV];
 #pragma unroll
-          for (int c = 0; c < CW; ++c) row[S::dim + c] = qs[c];
+    for (int a = 0; a < TPL * CW; ++a) part[a] = 0;
+    for (int sub = 0; sub < cnt; sub += SUB) {
+      // only the block's valid rows are evaluated; table rows past them are never summed
+      const int nj = min(SUB, cnt - sub);
+#pragma unroll UNR
+      for (int jj = 0; jj < nj; ++jj) {
+        const R* row = tw + (sub + jj) * NV;
+        const R xs = row[0];
+        const R ys = row[1];
+        const R zs = (S::dim == 3) ? row[2] : R(0);
+        R r0 = 0, r1 = 0;
+#pragma unroll
+        for (int k = 0; k < TPL; ++k) {
+          const R dx = xt[k] - xs;
+          const R dy = yt[k] - ys;
+          const R dz = (S::dim == 3) ? zt[k] - zs : R(0);
+          const Cplx<R> kv = kernel_value<R, OP, FT>(dx, dy, dz, R(0), kappa);
+          if (CW == 1) {
+            part[k] += kv.re * row[S::dim];
+            r0 += kv.re * qt[k];
+          } else {
+            // complex multiply-accumulates as two FMAs each (no separate product and add)
+            const R sr = row[S::dim], si = row[S::dim + 1];
+            part[2 * k] = fma(kv.re, sr, fma(-kv.im, si, part[2 * k]));
+            part[2 * k + 1] = fma(kv.re, si, fma(kv.im, sr, part[2 * k + 1]));
+            const R tr = qt[2 * k], ti = qt[2 * k + 1];
+            r0 = fma(kv.re, tr, fma(-kv.im, ti, r0));
+            r1 = fma(kv.re, ti, fma(kv.im, tr, r1));
+          }
+        }
+        rvl[((wave * SUB + jj) * CW + 0) * 64 + lane] = r0;
+        if (CW == 2) rvl[((wave * SUB + jj) * CW + CW - 1) * 64 + lane] = r1;
+      }
+      wave_sync_lds();
+      // column sums: lanes j * LPS .. j * LPS + LPS - 1 hold source j's partial over 64 / LPS lanes
+      const int j = lane / LPS, pp = lane % LPS;
+      constexpr int PER = 64 / LPS;
+#pragma unroll
+      for (int c = 0; c < CW; ++c) {
+        R v = 0;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) v += rvl[((wave * SUB + j) * CW + c) * 64 + pp * PER + i];
+#pragma unroll
+        for (int o = 1; o < LPS; o <<= 1) v += __shfl_xor(v, o, 64);
+        if (pp == 0 && sub + j < cnt) dst(wb + sub + j, c, double(v));
+      }
+      wave_sync_lds();
+    }
+  };
+  if constexpr (sizeof(R) == 8 && !OpT<OP>::cplx) etab = false;  // entry by entry (see the kernel)
+  if (etab) {
+    // sources as one stream across the item's entries, cut into tiles of 256 (small skeleton
+    // entries share a tile); each staged source carries its entry's sign in its charges and
+    // its reverse-sum slot (complemented for subtracted entries) in sidx
+    int ce = 0;  // entry of the tile start (uniform)
+    for (int base = 0; base < tot; base += kWG) {
+      const int nsc = min(kWG, tot - base);
+      {
+        const int pos = base + tid;
+        R* row = &tile[tid * NV];
+        if (pos < tot) {
+          int e = ce;
+          while (et->start[e + 1] <= pos) ++e;
+          const int j = pos - et->start[e];
+          const int sl = et->pt[e] + j, qs0 = et->q[e] + j;
+          const bool neg = (et->fl[e] & SKF_SRC_NEG) != 0;
+          row[0] = PREC ? px[sl] - et->sh[e][0] : px[sl];
+          row[1] = PREC ? py[sl] - et->sh[e][1] : py[sl];
+          if (S::dim == 3) row[2] = PREC ? pz[sl] - et->sh[e][2] : pz[sl];
+          const R* qs = qv + (long long)qs0 * w;
+#pragma unroll
+          for (int c = 0; c < CW; ++c) row[S::dim + c] = neg ? -qs[c] : qs[c];
+          sidx[tid] = neg ? ~qs0 : qs0;
         } else {
 #pragma unroll
           for (int c = 0; c < NV; ++c) row[c] = R(0);
         }
       }
       __syncthreads();
-      const int wb = base + wave * 64;  // wave-uniform
-      const int cnt = __builtin_amdgcn_readfirstlane(max(0, min(64, ns - wb)));
-      const R* tw = &tile[wave * 64 * NV];
-      R part[TPL * CW];
+      while (ce < nent && et->start[ce + 1] <= base + kWG) ++ce;
+      tile_eval(nsc, [&](int jt, int c, double v) {
+        const int ix = sidx[jt];
+        unsafeAtomicAdd(&d.urev[(long long)(ix < 0 ? ~ix : ix) * w + c], ix < 0 ? -v : v);
+      });
 #pragma unroll
-      for (int a = 0; a < TPL * CW; ++a) part[a] = 0;
-      for (int sub = 0; sub < cnt; sub += SUB) {
-        // only the block's valid rows are evaluated; table rows past them are never summed
-        const int nj = min(SUB, cnt - sub);
-#pragma unroll UNR
-        for (int jj = 0; jj < nj; ++jj) {
-          const R* row = tw + (sub + jj) * NV;
-          const R xs = row[0];
-          const R ys = row[1];
-          const R zs = (S::dim == 3) ? row[2] : R(0);
-          R r0 = 0, r1 = 0;
-#pragma unroll
-          for (int k = 0; k < TPL; ++k) {
-            const R dx = xt[k] - xs;
-            const R dy = yt[k] - ys;
-            const R dz = (S::dim == 3) ? zt[k] - zs : R(0);
-            const Cplx<R> kv = kernel_value<R, OP, FT>(dx, dy, dz, R(0), kappa);
-            if (CW == 1) {
-              part[k] += kv.re * row[S::dim];
-              r0 += kv.re * qt[k];
-            } else {
-              // complex multiply-accumulates as two FMAs each (no separate product and add)
-              const R sr = row[S::dim], si = row[S::dim + 1];
-              part[2 * k] = fma(kv.re, sr, fma(-kv.im, si, part[2 * k]));
-              part[2 * k + 1] = fma(kv.re, si, fma(kv.im, sr, part[2 * k + 1]));
-              const R tr = qt[2 * k], ti = qt[2 * k + 1];
-              r0 = fma(kv.re, tr, fma(-kv.im, ti, r0));
-              r1 = fma(kv.re, ti, fma(kv.im, tr, r1));
-            }
-          }
-          rvl[((wave * SUB + jj) * CW + 0) * 64 + lane] = r0;
-          if (CW == 2) rvl[((wave * SUB + jj) * CW + CW - 1) * 64 + lane] = r1;
-        }
-        wave_sync_lds();
-        // column sums: lanes j * LPS .. j * LPS + LPS - 1 hold source j's partial over 64 / LPS lanes
-        const int j = lane / LPS, pp = lane % LPS;
-        constexpr int PER = 64 / LPS;
-#pragma unroll
-        for (int c = 0; c < CW; ++c) {
-          R v = 0;
-#pragma unroll
-          for (int i = 0; i < PER; ++i) v += rvl[((wave * SUB + j) * CW + c) * 64 + pp * PER + i];
-#pragma unroll
-          for (int o = 1; o < LPS; o <<= 1) v += __shfl_xor(v, o, 64);
-          if (pp == 0 && sub + j < cnt)
-            unsafeAtomicAdd(&d.urev[(long long)(qo + wb + sub + j) * w + c], sgn * double(v));
-        }
-        wave_sync_lds();
-      }
-#pragma unroll
-      for (int a = 0; a < TPL * CW; ++a) acc[a] += sgn * double(part[a]);
+      for (int a = 0; a < TPL * CW; ++a) acc[a] += double(part[a]);
       __syncthreads();
+    }
+  } else {
+    for (int s = sb; s < se; ++s) {
+      const int ps = d.e_pt[s], ns = d.e_n[s], qo = d.e_q[s], fl = d.e_flags[s];
+      const double sgn = (fl & SKF_SRC_NEG) ? -1.0 : 1.0;
+      // precision form: sources are shifted into the target box's frame (minus the difference of
+      // the centres) as they are staged, once per source for all targets of the item
+      R ox = 0, oy = 0, oz = 0;
+      if (PREC) {
+        const int eb = d.e_box[s];
+        ox = R(ctx - d.centers[3 * eb + 0]);
+        oy = R(cty - d.centers[3 * eb + 1]);
+        oz = R(ctz - d.centers[3 * eb + 2]);
+      }
+      for (int base = 0; base < ns; base += kWG) {
+        const int nsc = min(kWG, ns - base);
+        {
+          const int j = base + tid;
+          R* row = &tile[tid * NV];
+          if (j < ns) {
+            const int sl = ps + j;
+            row[0] = PREC ? px[sl] - ox : px[sl];
+            row[1] = PREC ? py[sl] - oy : py[sl];
+            if (S::dim == 3) row[2] = PREC ? pz[sl] - oz : pz[sl];
+            const R* qs = qv + (long long)(qo + j) * w;
+#pragma unroll
+            for (int c = 0; c < CW; ++c) row[S::dim + c] = qs[c];
+          } else {
+#pragma unroll
+            for (int c = 0; c < NV; ++c) row[c] = R(0);
+          }
+        }
+        __syncthreads();
+        tile_eval(nsc, [&](int jt, int c, double v) {
+          unsafeAtomicAdd(&d.urev[(long long)(qo + base + jt) * w + c], sgn * v);
+        });
+#pragma unroll
+        for (int a = 0; a < TPL * CW; ++a) acc[a] += sgn * double(part[a]);
+        __syncthreads();
+      }
     }
   }
   // forward sums: combine the 4 waves, one target slot at a time
@@ -241,6 +302,8 @@ __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
   __shared__ R tile[kWG * NV];
   __shared__ R rvl[4 * SUB * CW * 64];  // reverse partials of one block of sources, per wave
   __shared__ double red[4 * kTCH * CW];
+  __shared__ SymEnt<R> et;
+  __shared__ int sidx[kWG];  // reverse-sum slot of each staged source (complemented if subtracted)
 
   const int4 it = d.items[blockIdx.x];
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
@@ -252,19 +315,70 @@ __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
     cty = d.centers[3 * gb + 1];
     ctz = d.centers[3 * gb + 2];
   }
+  const int nent = se - sb;
+  // the source stream across entries pays where entries are small against the 4 x 64 tile; the
+  // FP64 real kernels stage entry by entry (measured faster there: the stream path's registers
+  // halve their occupancy)
+  constexpr bool STREAM = sizeof(R) == 4 || OpT<OP>::cplx;
+  const bool etab = STREAM && nent <= kSymEnt;  // uniform
+  int tot = 0;
+  if (etab) {
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (tid < 64) {
+      // wave 0: prefix sums of the entry sizes, two per lane
+      int n2[2], loc = 0;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int e = lane * 2 + u;
+        n2[u] = e < nent ? d.e_n[sb + e] : 0;
+        loc += n2[u];
+      }
+      int inc = loc;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+      }
+      int run = inc - loc;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        et.start[lane * 2 + u] = run;
+        run += n2[u];
+      }
+      if (lane == 63) et.start[kSymEnt] = run;
+    }
+    for (int e = tid; e < nent; e += kWG) {
+      const int s = sb + e;
+      et.pt[e] = d.e_pt[s];
+      et.q[e] = d.e_q[s];
+      et.fl[e] = d.e_flags[s];
+      R ox = 0, oy = 0, oz = 0;
+      if (PREC) {
+        const int eb = d.e_box[s];
+        ox = R(ctx - d.centers[3 * eb + 0]);
+        oy = R(cty - d.centers[3 * eb + 1]);
+        oz = R(ctz - d.centers[3 * eb + 2]);
+      }
+      et.sh[e][0] = ox;
+      et.sh[e][1] = oy;
+      et.sh[e][2] = oz;
+    }
+    __syncthreads();
+    tot = et.start[nent];
+  }
   const int tend = min(ng, t0 + kSymTargets);
   // wave-uniform passes; the last one takes as many targets per lane as it needs (no idle slots
   // beyond the next multiple of 64)
   for (int tp = t0; tp < tend;) {
     const int tpl = min(TPL, (tend - tp + 63) / 64);
     if (tpl >= 4)
-      sym_pass<R, OP, PREC, UNR, 4, FT>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+      sym_pass<R, OP, PREC, UNR, 4, FT>(d, tile, rvl, red, sidx, &et, etab, nent, tot, tp, tend, t0, sb, se, ctx, cty, ctz);
     else if (tpl == 3)
-      sym_pass<R, OP, PREC, UNR, 3, FT>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+      sym_pass<R, OP, PREC, UNR, 3, FT>(d, tile, rvl, red, sidx, &et, etab, nent, tot, tp, tend, t0, sb, se, ctx, cty, ctz);
     else if (tpl == 2)
-      sym_pass<R, OP, PREC, UNR, 2, FT>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+      sym_pass<R, OP, PREC, UNR, 2, FT>(d, tile, rvl, red, sidx, &et, etab, nent, tot, tp, tend, t0, sb, se, ctx, cty, ctz);
     else
-      sym_pass<R, OP, PREC, UNR, 1, FT>(d, tile, rvl, red, tp, tend, t0, sb, se, ctx, cty, ctz);
+      sym_pass<R, OP, PREC, UNR, 1, FT>(d, tile, rvl, red, sidx, &et, etab, nent, tot, tp, tend, t0, sb, se, ctx, cty, ctz);
     tp += 64 * tpl;
   }
 }
