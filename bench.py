@@ -55,7 +55,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # rocprofv3 --pmc passes summarised in profiles/apply_laplace3d_uniform1M_hbm_traffic.md
 # (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, scale factors from a 512 MiB copy of the same run)
 P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400, False): 2549.1e6 + 14.2e6,
-                          (1_000_000, 1e-6, 400, True): 593.5e6 + 635.7e6}
+                          (1_000_000, 1e-6, 400, True): 641.3e6 + 631.0e6}
 
 
 def log(msg):
