@@ -29,7 +29,7 @@ import torch
 from . import hip, operators as operators_mod
 from .layout import build_layout, merge_layouts
 
-__all__ = ["SkelFMMBatched", "BatchedInstances"]
+__all__ = ["SkelFMMBatched", "BatchedInstances", "CapturedApply"]
 
 
 class SkelFMMBatched(torch.nn.Module):
@@ -49,6 +49,7 @@ class SkelFMMBatched(torch.nn.Module):
                 "the HIP apply supports the built-in operators of skelfmm_amd.operators only")
         self._plans = {}
         self._graphs = {}
+        self.generation = 0
 
     # ------------------------------------------------------------------ build
     @classmethod
@@ -70,6 +71,7 @@ class SkelFMMBatched(torch.nn.Module):
         self.kernel_backend = "hip"
         self._plans = {}
         self._graphs = {}
+        self.generation = 0
         return self.upload_layout(L, use_single_precision, device)
 
     def save(self, path, *, points=None, geometry=None, leaf_size=-1, extra=None, extra_arrays=None):
@@ -197,10 +199,17 @@ class SkelFMMBatched(torch.nn.Module):
         # bound of every pair distance: the diameter of the points' bounding box
         P = np.asarray(L.points)
         self.max_distance = float(np.linalg.norm(P.max(axis=1) - P.min(axis=1))) if P.size else 0.0
+        self._invalidate()
+        return self
+
+    def _invalidate(self):
+        """Drop plans and cached graphs after a change of layout or kernel settings. Graphs
+        already handed out (``capture``) keep their own buffers alive; ``generation`` tells a
+        caller holding one that it no longer matches this object's settings."""
         for k in list(self._plans):
             self._destroy_plan(k)
         self._graphs = {}
-        return self
+        self.generation = getattr(self, "generation", 0) + 1
 
     def _fill_split_structs(self):
         for st, t in zip(self._split_structs, self._split):
@@ -213,16 +222,12 @@ class SkelFMMBatched(torch.nn.Module):
     def set_sym_unroll(self, n):
         """Sources per inner step of the symmetric near-field kernel (1, 2 or 4; 0 = default 2)."""
         self.sym_unroll = int(n)
-        for k in list(self._plans):
-            self._destroy_plan(k)
-        self._graphs = {}
+        self._invalidate()
 
     def set_symmetric(self, on):
         """Use (True, default when available) or bypass the symmetric near-field kernel."""
         self.use_symmetric = bool(on) and self._split is not None
-        for k in list(self._plans):
-            self._destroy_plan(k)
-        self._graphs = {}
+        self._invalidate()
 
     def _p2p_args(self, width, urev=None):
         a = hip.P2PArgs()
@@ -362,9 +367,7 @@ class SkelFMMBatched(torch.nn.Module):
                 t["items"] = torch.as_tensor(np.ascontiguousarray(part.items.reshape(-1), dtype=np.int32),
                                              device=self.device)
             self._fill_split_structs()
-        for k in list(self._plans):
-            self._destroy_plan(k)
-        self._graphs = {}
+        self._invalidate()
 
     def time_near_field(self, width, reps=5, part="all"):
         """Mean duration (ms) of one near-field launch for ``width`` channels, HIP events on the current stream.
@@ -400,16 +403,26 @@ class SkelFMMBatched(torch.nn.Module):
     def set_mfma_min_channels(self, n):
         """Channel count from which FP32 real kernels use the matrix-core near field (0 default 16, <0 never)."""
         self.mfma_min_channels = int(n)
-        for k in list(self._plans):
-            self._destroy_plan(k)
-        self._graphs = {}
+        self._invalidate()
 
     # ------------------------------------------------------------ HIP graphs
+    def _device_buffers(self):
+        """Every device tensor an apply's kernels address (layout tables, points, T)."""
+        keep = [self.T, self.XX_slots, self.normals, self.centers, self.leaf_slot, self.leaf_user]
+        keep.append(dict(self._nf))
+        keep.extend(dict(t) for t in (self._split or []))
+        keep.extend(dict(t) for t in self._stages)
+        return keep
+
     def capture(self, x, out):
-        """Capture one apply ``x -> out`` (fixed device buffers) into a HIP graph and return it.
+        """Capture one apply ``x -> out`` (fixed device buffers) into a HIP graph.
 
         The apply issues only kernels (its zero fills are kernels, not memset nodes), so the
-        graph replays the whole apply with one launch call: ``graph.replay()``.
+        graph replays the whole apply with one launch call: ``captured.replay()``. The returned
+        ``CapturedApply`` holds the plan's work vectors and the layout tables the graph's
+        kernels address, so a later setter (``set_symmetric``, ``set_near_field_split``, ...)
+        or a new layout cannot free memory under it; ``captured.stale`` then turns True and
+        a caller that wants the new settings captures again.
         """
         if x.shape[0] != self.N:
             raise ValueError("charge vector length must match the number of points")
@@ -418,10 +431,12 @@ class SkelFMMBatched(torch.nn.Module):
         with torch.cuda.stream(side):
             self.apply_device(x, out=out)  # creates the plan and its work vectors outside the capture
         torch.cuda.current_stream(self.device).wait_stream(side)
+        xr = torch.view_as_real(x) if x.is_complex() else x
+        plan = self._plans[int(np.prod(xr.shape[1:])) if xr.dim() > 1 else 1]
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self.apply_device(x, out=out)
-        return g
+        return CapturedApply(g, self, [plan, x, out] + self._device_buffers())
 
     # ------------------------------------------------------------ accounting
     def nbytes_proj(self) -> int:
@@ -439,6 +454,28 @@ class SkelFMMBatched(torch.nn.Module):
                 acc += t.numel() * t.element_size()
         acc += self.leaf_slot.numel() * 4 + self.leaf_user.numel() * 4
         return int(acc)
+
+
+class CapturedApply:
+    """A HIP graph of one apply together with every device buffer its kernels use.
+
+    ``replay()`` launches the graph. ``stale`` is True once the apply object it came from
+    has changed its layout or kernel settings (its ``generation`` moved on); the graph then
+    still replays safely, with the settings it was captured with.
+    """
+
+    def __init__(self, graph, owner, buffers):
+        self.graph = graph
+        self.generation = owner.generation
+        self._owner = owner
+        self._buffers = buffers
+
+    @property
+    def stale(self):
+        return self._owner.generation != self.generation
+
+    def replay(self):
+        self.graph.replay()
 
 
 class BatchedInstances(SkelFMMBatched):
@@ -467,6 +504,7 @@ class BatchedInstances(SkelFMMBatched):
         self.kernel_backend = "hip"
         self._plans = {}
         self._graphs = {}
+        self.generation = 0
         self.instance_sizes = [int(b.N) for b in builds]
         self.ninstances = len(builds)
         L = merge_layouts([build_layout(b, max_sources=max_sources) for b in builds])

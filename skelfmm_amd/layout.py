@@ -515,3 +515,95 @@ def _near_field(L, fmm, off, bs, rk, max_sources):
             assert one.interactions + 2 * sym.interactions == L.interactions
             L.split = (one, sym)
             L.evaluations = one.interactions + sym.interactions
+
+
+def _fail(what):
+    raise ValueError(f"inconsistent box layout: {what}")
+
+
+def _within(name, a, lo, hi):
+    """Every entry of ``a`` in ``[lo, hi)`` (``hi`` may be an array of the same length)."""
+    a = np.asarray(a, dtype=np.int64)
+    if a.size and (a.min() < lo or np.any(a >= hi)):
+        _fail(f"{name} outside [{lo}, {hi if np.isscalar(hi) else 'limit'})")
+
+
+def _lengths(name, n, *arrays):
+    for a in arrays:
+        if a is None or np.asarray(a).shape[0] != n:
+            _fail(f"{name}: tables of different lengths")
+
+
+def _check_lists(name, P, nslots, nboxes):
+    ng, ne = int(np.asarray(P.g_pt).shape[0]), int(np.asarray(P.e_pt).shape[0])
+    _lengths(name + " groups", ng, P.g_pt, P.g_n, P.g_out, P.g_box, P.g_e0, P.g_e1)
+    _lengths(name + " entries", ne, P.e_pt, P.e_n, P.e_q, P.e_box, P.e_flags)
+    g_pt, g_n, g_out = (np.asarray(a, dtype=np.int64) for a in (P.g_pt, P.g_n, P.g_out))
+    e_pt, e_n, e_q = (np.asarray(a, dtype=np.int64) for a in (P.e_pt, P.e_n, P.e_q))
+    g_e0, g_e1 = np.asarray(P.g_e0, dtype=np.int64), np.asarray(P.g_e1, dtype=np.int64)
+    _within(name + ".g_n", g_n, 0, nslots + 1)
+    _within(name + ".g_pt", g_pt, 0, nslots - g_n + 1)
+    _within(name + ".g_out", g_out, 0, 2 * nslots - g_n + 1)
+    _within(name + ".g_box", P.g_box, 0, nboxes)
+    _within(name + ".e_n", e_n, 0, nslots + 1)
+    _within(name + ".e_pt", e_pt, 0, nslots - e_n + 1)
+    _within(name + ".e_q", e_q, 0, 2 * nslots - e_n + 1)
+    _within(name + ".e_box", P.e_box, 0, nboxes)
+    _within(name + ".e_flags", P.e_flags, 0, SRC_NEG + SRC_SELF + 1)
+    _within(name + ".g_e0", g_e0, 0, ne + 1)
+    if ng and (np.any(g_e1 < g_e0) or np.any(g_e1 > ne)):
+        _fail(f"{name}: group entry ranges outside the entry table")
+    it = np.asarray(P.items, dtype=np.int64).reshape(-1, 4)
+    if it.shape[0]:
+        _within(name + " item group", it[:, 0], 0, ng)
+        g = it[:, 0]
+        if (np.any(it[:, 1] < 0) or np.any(it[:, 1] >= np.maximum(g_n[g], 1))
+                or np.any(it[:, 2] < g_e0[g]) or np.any(it[:, 3] > g_e1[g]) or np.any(it[:, 2] > it[:, 3])):
+            _fail(f"{name}: a work item leaves its group's targets or entries")
+
+
+def check_layout(L):
+    """Raise ``ValueError`` unless every index of layout ``L`` stays inside the arrays it indexes.
+
+    The device kernels trust these tables (slot ranges, charge/potential offsets, T offsets,
+    gather indices, work items); a layout read from a file is checked here before upload."""
+    ndim, N, nboxes, nslots = int(L.ndim), int(L.N), int(L.nboxes), int(L.nslots)
+    if ndim not in (2, 3) or min(N, nboxes, nslots) < 0:
+        _fail("bad dimension or sizes")
+    pts = np.asarray(L.points)
+    if pts.shape != (ndim, nslots):
+        _fail(f"points shape {pts.shape}, expected {(ndim, nslots)}")
+    if L.normals is not None and np.asarray(L.normals).shape != (3, nslots):
+        _fail("normals shape")
+    if np.asarray(L.centers).shape != (nboxes, 3):
+        _fail("centers shape")
+    _lengths("slot tables", nslots, L.slot_box, L.slot_point)
+    _within("slot_box", L.slot_box, 0, nboxes)
+    _within("slot_point", L.slot_point, 0, N)
+    _lengths("box offsets", nboxes, L.off, L.toff)
+    _within("off", L.off, -1, nslots)
+    _lengths("leaf tables", N, L.leaf_slot, L.leaf_user)
+    _within("leaf_slot", L.leaf_slot, 0, nslots)
+    _within("leaf_user", L.leaf_user, 0, N)
+    ntt = int(L.T.shape[0])
+    _within("toff", L.toff, 0, ntt + 1)
+    for direction in ("up", "down"):
+        for S in getattr(L, direction):
+            name = f"{direction} stage (level {S.level})"
+            nb = int(np.asarray(S.box_off).shape[0])
+            _lengths(name, nb, S.box_bs, S.box_rank, S.box_toff)
+            off, bs, rk, toff = (np.asarray(a, dtype=np.int64) for a in (S.box_off, S.box_bs, S.box_rank,
+                                                                         S.box_toff))
+            if nb and (np.any(rk < 0) or np.any(rk > bs) or np.any(off < 0) or np.any(off + bs > nslots)
+                       or np.any(toff < 0) or np.any(toff + rk * (bs - rk) > ntt)):
+                _fail(f"{name}: a box's slots or T block leave their arrays")
+            _lengths(name + " gathers", int(np.asarray(S.gather_dst).shape[0]), S.gather_src)
+            _within(name + " gather_dst", S.gather_dst, 0, nslots)
+            _within(name + " gather_src", S.gather_src, 0, nslots)
+            it = np.asarray(S.items, dtype=np.int64).reshape(-1, 2)
+            _within(name + " item box", it[:, 0], 0, nb)
+            _within(name + " item start", it[:, 1], 0, nslots + 1)
+    _check_lists("near field", L, nslots, nboxes)
+    for i, part in enumerate(getattr(L, "split", None) or ()):
+        _check_lists(f"split part {i}", part, nslots, nboxes)
+    return L

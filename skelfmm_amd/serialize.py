@@ -19,7 +19,9 @@ executes nothing from it):
 * ``extra/<name>``: per-box build tables the caller adds (``rank_list``, ``bs_list``).
 
 A file of another format name or a newer version is rejected with ``ValueError``; so is
-an array whose CRC-32 does not match (``verify=True``).
+an array whose CRC-32 does not match (``verify=True``), and so is a layout whose index tables
+leave the arrays they index (``layout.check_layout``, always run: the CRC lives in the same
+file, so it only catches accidental corruption).
 """
 
 from __future__ import annotations
@@ -30,7 +32,7 @@ from typing import NamedTuple
 
 import numpy as np
 
-from .layout import BoxLayout, LevelStage, NearLists
+from .layout import BoxLayout, LevelStage, NearLists, check_layout
 
 __all__ = ["FORMAT", "VERSION", "Serialized", "save_layout", "load_layout"]
 
@@ -146,4 +148,13 @@ def load_layout(path, verify=True):
             parts.append(part)
         L.split = tuple(parts)
     extra = {k[len("extra/"):]: v for k, v in arrays.items() if k.startswith("extra/")}
-    return Serialized(header, L, arrays["points"], arrays.get("geometry"), extra)
+    try:
+        check_layout(L)
+    except (AttributeError, TypeError, IndexError) as exc:  # a missing or malformed table
+        raise ValueError(f"{path}: incomplete box layout ({exc})") from exc
+    except ValueError as exc:
+        raise ValueError(f"{path}: {exc}") from exc
+    points = arrays["points"]
+    if points.shape != (L.N, L.ndim):
+        raise ValueError(f"{path}: points shape {points.shape}, expected {(L.N, L.ndim)}")
+    return Serialized(header, L, points, arrays.get("geometry"), extra)

@@ -63,7 +63,8 @@ class FMMOperator:
         if not self.graph:
             return self.dev.apply_device(q)
         key = tuple(q.shape)
-        if key not in self._graphs:
+        held = self._graphs.get(key)
+        if held is None or held[2].stale:  # first use, or the apply's settings changed since
             qb = torch.empty_like(q)
             yb = torch.empty_like(q)
             qb.copy_(q)

@@ -352,6 +352,47 @@ def test_graph_replay_matches_eager(single, nrhs):
     assert rel_l2(out.cpu().numpy(), 2.0 * eager.cpu().numpy()) < (1e-6 if single else 1e-14)
 
 
+def test_captured_graph_outlives_setters():
+    """A captured graph addresses the plan's work vectors and the layout's item tables. The
+    setters replace those; the graph must keep its own copies alive (replays stay correct and
+    write nothing into memory handed out since), report itself stale, and the solver operator
+    must capture again and then follow the new settings."""
+    from skelfmm_amd.batched import SkelFMMBatched
+    from skelfmm_amd.solvers import FMMOperator
+
+    X = np.random.default_rng(4).random((20000, 3))
+    fmm = make_build(X, ops.LAPLACE_3D, tol=1e-6, leaf_size=150)
+    dev = SkelFMMBatched(fmm.tree, ops.LAPLACE_3D).build_from_copy(fmm, use_single_precision=True)
+    assert dev.use_symmetric
+    x = torch.randn(X.shape[0], dtype=torch.float32, device=_gpu())
+    out = torch.empty_like(x)
+    eager = dev.apply_device(x).clone()
+    g = dev.capture(x, out)
+    A = FMMOperator(dev, graph=True)
+    y_graph = A(x)
+    gen0 = dev.generation
+    dev.set_near_field_split(1024)
+    dev.set_symmetric(False)
+    assert dev.generation > gen0 and g.stale
+    # memory freed by the setters would be handed out again here
+    fills = [torch.full((2 * dev.nslots,), 7.0, dtype=torch.float32, device=_gpu()) for _ in range(6)]
+    fills += [torch.full((int(n),), 3, dtype=torch.int32, device=_gpu()) for n in (dev._nf["items"].numel(),) * 4]
+    for _ in range(3):
+        out.zero_()
+        g.replay()
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), eager.cpu().numpy()) < 1e-6
+    for f in fills:
+        assert bool((f == f.reshape(-1)[0]).all())
+    # the solver operator notices the change and captures again
+    y_new = A(x)
+    assert not A._graphs[(X.shape[0],)][2].stale
+    ref = dev.apply_device(x)
+    torch.cuda.synchronize()
+    assert rel_l2(y_new.cpu().numpy(), ref.cpu().numpy()) < 1e-6
+    assert rel_l2(y_graph.cpu().numpy(), eager.cpu().numpy()) < 1e-6
+
+
 def test_fp32_double_layer_surface_conditioning():
     """Double layer on the unit sphere with normals = positions: rn = (x - y).n = -r^2/2 is
     ill-conditioned in the coordinates (rounding only the points to FP32 costs ~4e-3 in an exact

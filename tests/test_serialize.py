@@ -142,6 +142,65 @@ def test_detects_a_changed_byte():
     load_layout(bad, verify=False)  # still readable when the check is off
 
 
+def _plant(key, mutate):
+    """A copy of the first case's file with ``mutate(array)`` applied to ``key`` and the CRC
+    recomputed, as a hand-made or inconsistent file would carry it."""
+    from skelfmm_amd.serialize import _crc
+
+    _, f, kappa, geom = CASES[0]
+    _, buf = _roundtrip(f, kappa, geom)
+
+    planted = {}
+
+    def arrays_fn(arrays):
+        arrays[key] = arrays[key].copy()
+        mutate(arrays[key])
+        planted["crc"] = _crc(arrays[key])
+
+    def header_fn(header):
+        header["crc32"][key] = planted["crc"]
+
+    # _rewrite applies the array change first, then the header change
+    buf.seek(0)
+    with np.load(buf, allow_pickle=False) as z:
+        arrays_fn({key: z[key]})
+    return _rewrite(buf, mutate_header=header_fn, mutate_arrays=arrays_fn)
+
+
+def _set_last(v):
+    def f(a):
+        a.reshape(-1)[-1] = v
+    return f
+
+
+@pytest.mark.parametrize("key,mutate", [
+    ("layout/leaf_slot", _set_last(10 ** 8)),
+    ("layout/g_pt", _set_last(-1)),
+    ("layout/e_q", _set_last(2 ** 30)),
+    ("layout/e_box", _set_last(10 ** 6)),
+    ("layout/items", _set_last(10 ** 6)),            # entry range past the group's entries
+    ("layout/g_e1", _set_last(10 ** 7)),
+    ("up/1/gather_src", _set_last(10 ** 8)),
+    ("up/0/box_toff", _set_last(10 ** 12)),         # T block past the end of T
+    ("up/0/items", _set_last(-5)),
+    ("split/1/e_n", _set_last(10 ** 7)),
+    ("layout/toff", _set_last(-3)),
+], ids=lambda v: v if isinstance(v, str) else "")
+def test_rejects_out_of_range_indices(key, mutate):
+    """The CRC sits in the same file, so a consistent-looking file with an index that leaves
+    its array must still be refused before any table reaches the device."""
+    with pytest.raises(ValueError, match="inconsistent box layout"):
+        load_layout(_plant(key, mutate))
+
+
+def test_check_layout_accepts_every_built_layout():
+    from skelfmm_amd.layout import check_layout
+
+    for _, f, _, _ in CASES:
+        L = build_layout(f, max_sources=512)
+        assert check_layout(L) is L
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("single", [False, True])
 def test_saved_operator_applies_like_the_original(tmp_path, single):
