@@ -146,6 +146,12 @@ typedef struct skf_p2p_args {
      (FP32: hardware sine/cosine after the reduction, FP64: polynomials) when
      kappa * max_distance <= 65536 */
   double max_distance;
+  /* precision form of the double-layer kernels (required there, ignored otherwise): FP64 rows
+     [7][geo64_n] per slot: absolute x, y, z, normal nx, ny, nz, and n . x. The normal projection
+     rn = (x - y) . n of a pair is formed from these in FP64: on a surface rn ~ r^2 / 2 for close
+     pairs, far below the FP32 rounding of box-centred coordinates (~6e-8 of a box side) */
+  const double* geo64;
+  long long geo64_n;
 } skf_p2p_args;
 
 /* one level of the upward or downward interpolation pass */
@@ -185,6 +191,9 @@ typedef struct skf_plan skf_plan;
 
 const char* skf_last_error(void);
 int skf_version(void);
+/* 16 hex digits: hash of the sources and compile flags the library was built from (no reference
+   counterpart; lets a loader prove that the binary matches the source tree) */
+const char* skf_build_id(void);
 
 /* zero `n` real scalars of the given precision */
 int skf_fill_zero(void* ptr, long long n, int precision, void* stream);

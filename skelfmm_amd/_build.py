@@ -2,6 +2,7 @@
 
 from __future__ import annotations
 
+import hashlib
 import os
 import subprocess
 import sys
@@ -29,9 +30,45 @@ def hipcc():
     return cand if os.path.exists(cand) else "hipcc"
 
 
+# marker in the built file that precedes the source hash (found without loading the library)
+BUILD_TAG = b"skelfmm-build-id:"
+
+
+def source_files():
+    """Every file the library is compiled from: the HIP sources, their headers, the C ABI header."""
+    files = [os.path.join(HERE, "csrc", f) for f in os.listdir(os.path.join(HERE, "csrc"))
+             if f.endswith((".hip", ".h"))]
+    files += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")]
+    return sorted(files)
+
+
+def source_hash():
+    """16 hex digits of SHA-256 over the library's sources and compile flags."""
+    h = hashlib.sha256()
+    for path in source_files():
+        h.update(os.path.relpath(path, ROOT).encode() + b"\0")
+        with open(path, "rb") as fh:
+            h.update(fh.read())
+    h.update(" ".join(HIPCC_FLAGS).encode())
+    return h.hexdigest()[:16]
+
+
+def built_hash(path=OUT):
+    """The source hash embedded in a built library file, or None (no file, no tag)."""
+    if not os.path.exists(path):
+        return None
+    with open(path, "rb") as fh:
+        data = fh.read()
+    at = data.find(BUILD_TAG)
+    if at < 0:
+        return None
+    return data[at + len(BUILD_TAG):at + len(BUILD_TAG) + 16].decode("ascii", errors="replace")
+
+
 def build_library(verbose=False):
     tmp = OUT + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include")] + SRC + ["-lhiprtc", "-o", tmp]
+    tag = f'-DSKF_SOURCE_HASH="{source_hash()}"'
+    cmd = [hipcc()] + HIPCC_FLAGS + [tag, "-I", os.path.join(ROOT, "include")] + SRC + ["-lhiprtc", "-o", tmp]
     if verbose:
         print(" ".join(cmd))
     res = subprocess.run(cmd, capture_output=True, text=True)

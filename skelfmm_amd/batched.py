@@ -132,8 +132,17 @@ class SkelFMMBatched(torch.nn.Module):
             pts = torch.as_tensor(rel, dtype=torch.float32, device=dev)
         self.XX_slots = pts.contiguous()
         self.normals = None
+        self.geo64 = None
         if L.normals is not None:
             self.normals = torch.as_tensor(L.normals, dtype=rd, device=dev).contiguous()
+            if self.precision == 1:
+                # double layers in the precision form: the normal projection (x - y) . n is formed in
+                # FP64 from absolute coordinates, normals and n . x per slot (rows of 7 x nslots)
+                P64 = np.zeros((3, L.nslots))
+                P64[:self.ndim] = L.points
+                N64 = np.asarray(L.normals, dtype=np.float64)
+                geo = np.concatenate([P64, N64, (P64 * N64).sum(axis=0, keepdims=True)])
+                self.geo64 = torch.as_tensor(geo, dtype=torch.float64, device=dev).contiguous()
         self.centers = torch.as_tensor(L.centers, dtype=torch.float64, device=dev).contiguous()
         self.complex_t = bool(L.T.is_complex()) if torch.is_tensor(L.T) else bool(np.iscomplexobj(L.T))
         if self.complex_t:
@@ -244,6 +253,9 @@ class SkelFMMBatched(torch.nn.Module):
             a.ny = hip.ptr(self.normals[1])
             a.nz = hip.ptr(self.normals[2])
         a.centers = hip.ptr(self.centers)
+        if self.geo64 is not None:
+            a.geo64 = hip.ptr(self.geo64)
+            a.geo64_n = int(self.geo64.shape[1])
         nf = self._nf
         a.ngroups = int(nf["g_pt"].shape[0])
         for k in ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags", "items"):
@@ -408,7 +420,7 @@ class SkelFMMBatched(torch.nn.Module):
     # ------------------------------------------------------------ HIP graphs
     def _device_buffers(self):
         """Every device tensor an apply's kernels address (layout tables, points, T)."""
-        keep = [self.T, self.XX_slots, self.normals, self.centers, self.leaf_slot, self.leaf_user]
+        keep = [self.T, self.XX_slots, self.normals, self.geo64, self.centers, self.leaf_slot, self.leaf_user]
         keep.append(dict(self._nf))
         keep.extend(dict(t) for t in (self._split or []))
         keep.extend(dict(t) for t in self._stages)
@@ -444,6 +456,8 @@ class SkelFMMBatched(torch.nn.Module):
 
     def nbytes_lists(self) -> int:
         acc = self.XX_slots.numel() * self.XX_slots.element_size()
+        if self.geo64 is not None:
+            acc += self.geo64.numel() * 8
         for t in self._nf.values():
             acc += t.numel() * t.element_size()
         for part in self._split or []:

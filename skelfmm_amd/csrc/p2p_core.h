@@ -236,27 +236,79 @@ struct P2PDev {
   double kappa;
   double* urev;  // reverse outputs of the symmetric kernel (FP64; u itself in the FP64 form)
   int fast_trig;  // every kernel argument kappa r is within kTrigFastMax
+  // precision form of the double layers: FP64 rows [7][geo64_n] (x, y, z, nx, ny, nz, n.x)
+  const double* geo64;
+  long long geo64_n;
+};
+
+// Normal projection rn = (x - y) . n of the double layers in the precision form. On a surface
+// rn ~ r^2 / 2 for close pairs while the FP32 box-centred coordinates carry ~6e-8 of a box side
+// of rounding each, so rn is formed in FP64 from FP64 data and rounded once:
+//   source normal (geom 1): rn = x . n_y - (n_y . y)   target values x, source values (n_y, n_y . y)
+//   target normal (geom 2): rn = (n_x . x) - n_x . y   target values (n_x, n_x . x), source values y
+template <int OP>
+struct Geo64 {
+  static constexpr bool on = OpT<OP>::geom != 0;
+  // the four values a staged source carries
+  __device__ static inline void source(const double* g, long long n, int sl, double* v) {
+    if (OpT<OP>::geom == 1) {
+      v[0] = g[3 * n + sl];
+      v[1] = g[4 * n + sl];
+      v[2] = g[5 * n + sl];
+      v[3] = g[6 * n + sl];
+    } else {
+      v[0] = g[sl];
+      v[1] = g[n + sl];
+      v[2] = g[2 * n + sl];
+      v[3] = 0.0;
+    }
+  }
+  // the four values a target holds
+  __device__ static inline void target(const double* g, long long n, int sl, double* v) {
+    if (OpT<OP>::geom == 1) {
+      v[0] = g[sl];
+      v[1] = g[n + sl];
+      v[2] = g[2 * n + sl];
+      v[3] = 0.0;
+    } else {
+      v[0] = g[3 * n + sl];
+      v[1] = g[4 * n + sl];
+      v[2] = g[5 * n + sl];
+      v[3] = g[6 * n + sl];
+    }
+  }
+  __device__ static inline float rn(const double* t, const double* sv) {
+    double v;
+    if (OpT<OP>::geom == 1)
+      v = fma(t[0], sv[0], fma(t[1], sv[1], fma(t[2], sv[2], -sv[3])));
+    else
+      v = fma(-t[0], sv[0], fma(-t[1], sv[1], fma(-t[2], sv[2], t[3])));
+    return float(v);
+  }
 };
 
 __device__ inline void atomic_acc(double* p, double v) { unsafeAtomicAdd(p, v); }
 __device__ inline void atomic_acc(float* p, float v) { unsafeAtomicAdd(p, v); }
 
 // Values per staged source: coordinates, optional source normal, NCH charges (x2 if complex).
-template <int OP, int NCH>
+// G64: the precision form of a double layer, whose normal projection comes from FP64 rows staged
+// beside the tile (Geo64), so the tile holds no FP32 normals.
+template <int OP, int NCH, bool G64 = false>
 struct Stage {
   static constexpr int dim = OpT<OP>::dim;
   static constexpr int cw = OpT<OP>::cplx ? 2 : 1;
-  static constexpr int ng = OpT<OP>::geom == 1 ? 3 : 0;
+  static constexpr int ng = (OpT<OP>::geom == 1 && !G64) ? 3 : 0;
   static constexpr int nraw = dim + ng + NCH * cw;
   static constexpr int nv = (nraw + 1) & ~1;  // even -> 16-byte aligned rows
 };
 
 // Sum over `cnt` staged sources of one wave's tile for this lane's target. SELF skips the
 // source whose local index equals the target's (the self pair's zero diagonal).
-template <typename R, int OP, int NCH, bool SELF, bool FT>
+template <typename R, int OP, int NCH, bool SELF, bool FT, bool G64 = false>
 __device__ inline void tile_sum(const R* __restrict__ tw, int cnt, int self_k, R xs0, R ys0, R zs0, R gxt, R gyt,
-                                R gzt, R kappa, R* part) {
-  using S = Stage<OP, NCH>;
+                                R gzt, R kappa, R* part, const double* __restrict__ gw = nullptr,
+                                const double* t64 = nullptr) {
+  using S = Stage<OP, NCH, G64>;
   constexpr int NV = S::nv;
   constexpr int CW = S::cw;
 #pragma unroll 4
@@ -266,8 +318,12 @@ __device__ inline void tile_sum(const R* __restrict__ tw, int cnt, int self_k, R
     R dy = ys0 - row[1];
     R dz = (S::dim == 3) ? zs0 - row[2] : R(0);
     R rn = 0;
-    if (OpT<OP>::geom == 1) rn = dx * row[S::dim] + dy * row[S::dim + 1] + dz * row[S::dim + 2];
-    if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
+    if constexpr (G64) {
+      rn = R(Geo64<OP>::rn(t64, gw + 4 * k));
+    } else {
+      if (OpT<OP>::geom == 1) rn = dx * row[S::dim] + dy * row[S::dim + 1] + dz * row[S::dim + 2];
+      if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
+    }
     Cplx<R> kv = kernel_value<R, OP, FT>(dx, dy, dz, rn, kappa);
     if (SELF && k == self_k) {
       kv.re = 0;
@@ -292,12 +348,14 @@ __device__ inline void tile_sum(const R* __restrict__ tw, int cnt, int self_k, R
 // partial sums are combined through LDS and added to the output with atomics.
 template <typename R, int OP, int NCH, bool PREC, bool FT = false>
 __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
-  using S = Stage<OP, NCH>;
+  constexpr bool G64 = PREC && Geo64<OP>::on;
+  using S = Stage<OP, NCH, G64>;
   constexpr int NV = S::nv;
   constexpr int CW = S::cw;
   constexpr int NA = NCH * CW;
   __shared__ R tile[kWG * NV];
   __shared__ double red[4][kTCH][NA];
+  __shared__ double gtile[G64 ? kWG * 4 : 1];  // FP64 normal-projection data of the staged sources
 
   const int4 it = d.items[blockIdx.x];
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
@@ -317,11 +375,13 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
   const R* qv = static_cast<const R*>(d.q);
   R xt = px[tslot], yt = py[tslot], zt = (S::dim == 3) ? pz[tslot] : R(0);
   R gxt = 0, gyt = 0, gzt = 0;
-  if (OpT<OP>::geom == 2) {
+  if (OpT<OP>::geom == 2 && !G64) {
     gxt = static_cast<const R*>(d.nx)[tslot];
     gyt = static_cast<const R*>(d.ny)[tslot];
     gzt = static_cast<const R*>(d.nz)[tslot];
   }
+  double t64[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (G64) Geo64<OP>::target(d.geo64, d.geo64_n, tslot, t64);
   double ctx = 0, cty = 0, ctz = 0;
   if (PREC) {
     const int gb = d.g_box[g];
@@ -365,9 +425,14 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
           const R* qs = qv + (long long)(qo + j) * d.width + ch0 * CW;
 #pragma unroll
           for (int c = 0; c < NA; ++c) row[S::dim + S::ng + c] = (c < nchv * CW) ? qs[c] : R(0);
+          if constexpr (G64) Geo64<OP>::source(d.geo64, d.geo64_n, sl, &gtile[tid * 4]);
         } else {
 #pragma unroll
           for (int c = 0; c < NV; ++c) row[c] = R(0);
+          if constexpr (G64) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) gtile[tid * 4 + c] = 0.0;
+          }
         }
       }
       __syncthreads();
@@ -377,11 +442,12 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
 #pragma unroll
       for (int a = 0; a < NA; ++a) part[a] = 0;
       const R* tw = &tile[wave * 64 * NV];
+      const double* gw = &gtile[G64 ? wave * 64 * 4 : 0];
       // the diagonal of a self segment falls in this tile only for targets tloc in [wb, wb + cnt)
       if (self && tloc - wb >= 0 && tloc - wb < 64)
-        tile_sum<R, OP, NCH, true, FT>(tw, cnt, tloc - wb, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part);
+        tile_sum<R, OP, NCH, true, FT, G64>(tw, cnt, tloc - wb, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part, gw, t64);
       else
-        tile_sum<R, OP, NCH, false, FT>(tw, cnt, -1, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part);
+        tile_sum<R, OP, NCH, false, FT, G64>(tw, cnt, -1, xs0, ys0, zs0, gxt, gyt, gzt, kappa, part, gw, t64);
 #pragma unroll
       for (int a = 0; a < NA; ++a) acc[a] += sgn * double(part[a]);
       __syncthreads();

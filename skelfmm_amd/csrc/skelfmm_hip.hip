@@ -46,6 +46,15 @@ static int set_err(int code, const char* fmt, ...) {
 extern "C" const char* skf_last_error(void) { return g_err; }
 extern "C" int skf_version(void) { return 1; }
 
+// hash of the sources and flags this library was compiled from (set by the build script), also
+// kept as a tagged string in the file so a loader can check it before loading
+#ifndef SKF_SOURCE_HASH
+#define SKF_SOURCE_HASH "0000000000000000"
+#endif
+extern "C" __attribute__((used, visibility("default"))) const char skf_build_tag[] =
+    "skelfmm-build-id:" SKF_SOURCE_HASH;
+extern "C" const char* skf_build_id(void) { return SKF_SOURCE_HASH; }
+
 #include "p2p_core.h"
 
 namespace {
@@ -421,13 +430,15 @@ struct Mfma<double> {
 template <typename R, int OP, bool PREC>
 __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   static_assert(OpT<OP>::cplx == 0, "real kernels only");
-  using S = Stage<OP, 1>;
+  constexpr bool G64 = PREC && Geo64<OP>::on;  // double layer, precision form: rn from FP64 rows
+  using S = Stage<OP, 1, G64>;
   using MF = Mfma<R>;
   typedef typename MF::V V;
   constexpr int DIM = S::dim;
   constexpr int NG = S::ng;
   __shared__ R qt[64 * kQS];
   __shared__ R pt[64 * 8];   // x, y, z, nx, ny, nz per source (precision form: in the target box frame)
+  __shared__ double pg[G64 ? 64 * 4 : 1];  // FP64 normal-projection data per source (Geo64)
   __shared__ int pself[64];  // local index of a self-segment source (its diagonal target), else -1
   // the item's entries (up to kEnt): stream offsets and per-entry data, so that finding the entry
   // of a staged source is a walk through LDS rather than a chain of dependent global loads
@@ -459,11 +470,13 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   const R* qv = static_cast<const R*>(d.q);
   const R xt = px[tslot], yt = py[tslot], zt = (DIM == 3) ? pz[tslot] : R(0);
   R gxt = 0, gyt = 0, gzt = 0;
-  if (OpT<OP>::geom == 2) {
+  if (OpT<OP>::geom == 2 && !G64) {
     gxt = static_cast<const R*>(d.nx)[tslot];
     gyt = static_cast<const R*>(d.ny)[tslot];
     gzt = static_cast<const R*>(d.nz)[tslot];
   }
+  double t64[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (G64) Geo64<OP>::target(d.geo64, d.geo64_n, tslot, t64);
   double ctx = 0, cty = 0, ctz = 0;
   if (PREC) {
     const int gb = d.g_box[g];
@@ -536,6 +549,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   // during the matrix work was measured slower: the extra registers cost a wave per SIMD)
   const int sr = tid >> 2, cq = (tid & 3) * 16;
   R rq[16], rp[6];
+  double rg[4] = {0.0, 0.0, 0.0, 0.0};
   int rself = -1;
   auto stage_load = [&](int tbase, int nsrc) {
     if (sr < nsrc) {
@@ -582,6 +596,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
         rp[3] = NG ? static_cast<const R*>(d.nx)[sl] : R(0);
         rp[4] = NG ? static_cast<const R*>(d.ny)[sl] : R(0);
         rp[5] = NG ? static_cast<const R*>(d.nz)[sl] : R(0);
+        if constexpr (G64) Geo64<OP>::source(d.geo64, d.geo64_n, sl, rg);
         rself = (fl & SKF_SRC_SELF) ? j : -1;
       }
     } else {
@@ -590,6 +605,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
       for (int c = 0; c < 16; ++c) rq[c] = R(0);
       rp[0] = rp[1] = rp[2] = R(1e18);
       rp[3] = rp[4] = rp[5] = R(0);
+      rg[0] = rg[1] = rg[2] = rg[3] = 0.0;
       rself = -1;
     }
     // advance the cursor by the tile (uniform)
@@ -612,6 +628,10 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
       R* row = &pt[sr * 8];
 #pragma unroll
       for (int c = 0; c < 6; ++c) row[c] = rp[c];
+      if constexpr (G64) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) pg[sr * 4 + c] = rg[c];
+      }
       pself[sr] = rself;
     }
   };
@@ -633,8 +653,12 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
         const R* row = &pt[sj * 8];
         const R dx = xt - row[0], dy = yt - row[1], dz = (DIM == 3) ? zt - row[2] : R(0);
         R rn = 0;
-        if (OpT<OP>::geom == 1) rn = dx * row[3] + dy * row[4] + dz * row[5];
-        if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
+        if constexpr (G64) {
+          rn = R(Geo64<OP>::rn(t64, &pg[sj * 4]));
+        } else {
+          if (OpT<OP>::geom == 1) rn = dx * row[3] + dy * row[4] + dz * row[5];
+          if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
+        }
         R a = kernel_value<R, OP>(dx, dy, dz, rn, R(0)).re;
         if (pself[sj] == tloc) a = R(0);
         const R* qrow = &qt[sj * kQS + (lane & 15)];
@@ -687,7 +711,8 @@ __device__ inline void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 template <typename R, int OP, bool PREC>
 __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   static_assert(OpT<OP>::cplx == 0, "real kernels only");
-  using S = Stage<OP, 1>;
+  constexpr bool G64 = PREC && Geo64<OP>::on;  // double layer, precision form: rn from FP64 rows
+  using S = Stage<OP, 1, G64>;
   using MF = Mfma<R>;
   typedef typename MF::V V;
   constexpr int DIM = S::dim;
@@ -702,6 +727,8 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   __shared__ __attribute__((aligned(16))) R qb[TR * kMB];
   __shared__ R pa[TR * PS];
   __shared__ R pb[TR * PS];
+  __shared__ double g64a[G64 ? TR * 4 : 1];  // FP64 normal-projection data per source (Geo64)
+  __shared__ double g64b[G64 ? TR * 4 : 1];
   __shared__ int sa[TR];
   __shared__ int sbf[TR];
   constexpr int kEnt = 128;
@@ -730,11 +757,13 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   const R* qv = static_cast<const R*>(d.q);
   const R xt = px[tslot], yt = py[tslot], zt = (DIM == 3) ? pz[tslot] : R(0);
   R gxt = 0, gyt = 0, gzt = 0;
-  if (OpT<OP>::geom == 2) {
+  if (OpT<OP>::geom == 2 && !G64) {
     gxt = static_cast<const R*>(d.nx)[tslot];
     gyt = static_cast<const R*>(d.ny)[tslot];
     gzt = static_cast<const R*>(d.nz)[tslot];
   }
+  double t64[4] = {0.0, 0.0, 0.0, 0.0};
+  if constexpr (G64) Geo64<OP>::target(d.geo64, d.geo64_n, tslot, t64);
   // precision form: sources are shifted into the target box's frame (the FP64 form has no centres)
   double ctx = 0, cty = 0, ctz = 0;
   if (PREC) {
@@ -845,6 +874,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   const int sr = tid >> 2;
   const bool meta_thread = (tid & 3) == 0 && sr < TR;
   R rp[PS], rsh[3];
+  double rg[4] = {0.0, 0.0, 0.0, 0.0};
   int rself = -1;
   auto stage_issue = [&](R* qdst, int tbase, int nsrc) {
     const R* src[4];
@@ -907,25 +937,31 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
         rsh[0] = ox;
         rsh[1] = oy;
         rsh[2] = oz;
+        if constexpr (G64) Geo64<OP>::source(d.geo64, d.geo64_n, sl, rg);
       } else {
         rp[0] = rp[1] = rp[2] = R(1e18);  // far away with sign 0: contributes exactly zero
         rsh[0] = rsh[1] = rsh[2] = R(0);
+        rg[0] = rg[1] = rg[2] = rg[3] = 0.0;
       }
     }
   };
-  auto meta_store = [&](R* __restrict__ pdst, int* __restrict__ sdst) {
+  auto meta_store = [&](R* __restrict__ pdst, int* __restrict__ sdst, double* __restrict__ gdst) {
     if (!meta_thread) return;
     pdst[sr * PS + 0] = rp[0] - rsh[0];
     pdst[sr * PS + 1] = rp[1] - rsh[1];
     pdst[sr * PS + 2] = rp[2] - rsh[2];
 #pragma unroll
     for (int c = 3; c < PS; ++c) pdst[sr * PS + c] = rp[c];
+    if constexpr (G64) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) gdst[sr * 4 + c] = rg[c];
+    }
     sdst[sr] = rself;
   };
   // matrix work of one staged tile
   // (restrict parameters: the reads carry alias scopes, so they need not wait for the loads in flight)
   auto compute = [&](const R* __restrict__ qcur, const R* __restrict__ pcur, const int* __restrict__ scur,
-                     int nsrc) {
+                     const double* __restrict__ gcur, int nsrc) {
     if (!wave_busy) return;
     if constexpr (FOLD) {
 #pragma unroll
@@ -938,8 +974,12 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
       const R* row = &pcur[sj * PS];
       const R dx = xt - row[0], dy = yt - row[1], dz = (DIM == 3) ? zt - row[2] : R(0);
       R rn = 0;
-      if (OpT<OP>::geom == 1) rn = dx * row[3] + dy * row[4] + dz * row[5];
-      if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
+      if constexpr (G64) {
+        rn = R(Geo64<OP>::rn(t64, &gcur[sj * 4]));
+      } else {
+        if (OpT<OP>::geom == 1) rn = dx * row[3] + dy * row[4] + dz * row[5];
+        if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
+      }
       R a = kernel_value<R, OP>(dx, dy, dz, rn, R(0)).re * row[SG];
       if (scur[sj] == tloc) a = R(0);
       const R* qrow = qcur + sj * kMB + (lane & 15);
@@ -954,7 +994,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
     }
   };
   // one tile: prefetch the next into (qn, pn, sn) while computing (qc, pc, sc)
-  auto step = [&](int base, R* qc, R* pc, int* sc, R* qn, R* pn, int* sn) {
+  auto step = [&](int base, R* qc, R* pc, int* sc, double* gc, R* qn, R* pn, int* sn, double* gn) {
     const int nsrc = min(TR, tot - base);
     const bool more = base + TR < tot;
     int nnext = 0;
@@ -965,14 +1005,14 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
     }
     // keep the staging ahead of the matrix work (the compiler may otherwise sink it below)
     asm volatile("" ::: "memory");
-    compute(qc, pc, sc, nsrc);
-    if (more) meta_store(pn, sn);
+    compute(qc, pc, sc, gc, nsrc);
+    if (more) meta_store(pn, sn, gn);
     wait_vm_loads();
     __syncthreads();
   };
   if (tot > 0) {
     stage_issue(qa, 0, min(TR, tot));
-    meta_store(pa, sa);
+    meta_store(pa, sa, g64a);
     __syncthreads();
   }
   // on every path: the target data loaded above is complete, so the tile loop's only
@@ -980,8 +1020,8 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   wait_vm_loads();
   __syncthreads();
   for (int base = 0; base < tot; base += 2 * TR) {
-    step(base, qa, pa, sa, qb, pb, sbf);
-    if (base + TR < tot) step(base + TR, qb, pb, sbf, qa, pa, sa);
+    step(base, qa, pa, sa, g64a, qb, pb, sbf, g64b);
+    if (base + TR < tot) step(base + TR, qb, pb, sbf, g64b, qa, pa, sa, g64a);
   }
   R* uo = static_cast<R*>(d.u);
 #pragma unroll
@@ -1011,16 +1051,18 @@ int launch_p2p_mfma(const P2PDev& d, int nitems, hipStream_t st) {
   return SKF_OK;
 }
 
-// static LDS of one p2p_kernel instance (tile + cross-wave reduction); kept within 64 KiB
-template <typename R, int OP, int NCH>
+// static LDS of one p2p_kernel instance (tile + cross-wave reduction + FP64 double-layer rows of
+// the precision form); kept within 64 KiB
+template <typename R, int OP, int NCH, bool PREC>
 constexpr int p2p_lds_bytes() {
-  using S = Stage<OP, NCH>;
-  return kWG * S::nv * int(sizeof(R)) + 4 * kTCH * NCH * S::cw * 8;
+  constexpr bool G64 = PREC && Geo64<OP>::on;
+  using S = Stage<OP, NCH, G64>;
+  return kWG * S::nv * int(sizeof(R)) + 4 * kTCH * NCH * S::cw * 8 + (G64 ? kWG * 4 * 8 : 8);
 }
 
 template <typename R, int OP, int NCH, bool PREC>
 int launch_p2p_t(const P2PDev& d, int nitems, hipStream_t st) {
-  if constexpr (p2p_lds_bytes<R, OP, NCH>() <= 65536) {
+  if constexpr (p2p_lds_bytes<R, OP, NCH, PREC>() <= 65536) {
     dim3 grid(nitems, (d.nch + NCH - 1) / NCH);
     if constexpr (HasFastTrig<R, OP>::value) {
       if (d.fast_trig) {
@@ -1162,6 +1204,8 @@ int run_p2p(const skf_p2p_args* a, const void* q, void* u, hipStream_t st) {
   if (!a->px || !a->py || (op_dim3(a->kernel) && !a->pz)) return set_err(SKF_ERR_ARG, "missing coordinates");
   if (op_geom(a->kernel) && (!a->nx || !a->ny || !a->nz)) return set_err(SKF_ERR_ARG, "missing normals");
   if (a->precision == 1 && !a->centers) return set_err(SKF_ERR_ARG, "precision form needs box centres");
+  if (a->precision == 1 && op_geom(a->kernel) && (!a->geo64 || a->geo64_n <= 0))
+    return set_err(SKF_ERR_ARG, "precision form of a double layer needs the FP64 geometry rows (geo64)");
   P2PDev d;
   d.px = a->px; d.py = a->py; d.pz = a->pz;
   d.nx = a->nx; d.ny = a->ny; d.nz = a->nz;
@@ -1174,6 +1218,8 @@ int run_p2p(const skf_p2p_args* a, const void* q, void* u, hipStream_t st) {
   d.nch = cx ? a->width / 2 : a->width;
   d.kappa = a->kappa;
   d.urev = nullptr;
+  d.geo64 = a->geo64;
+  d.geo64_n = a->geo64_n;
   // the 3D Helmholtz kernels take the branch-free reduced sincos (FP32: hardware sine/cosine after
   // the reduction; FP64: polynomials) when the host bounds every argument kappa r (kappa times
   // the largest pair distance) within its range

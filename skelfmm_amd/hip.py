@@ -21,6 +21,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libskelfmm_
 EXPORTED_SYMBOLS = (
     "skf_last_error",
     "skf_version",
+    "skf_build_id",
     "skf_fill_zero",
     "skf_gather_charges",
     "skf_scatter_result",
@@ -67,7 +68,7 @@ class P2PArgs(ctypes.Structure):
         ("nitems", _i), ("items", _vp), ("mfma_min_channels", _i),
         ("one", ctypes.POINTER(P2PLists)), ("sym", ctypes.POINTER(P2PLists)), ("urev", _vp),
         ("urev_len", ctypes.c_longlong), ("sym_unroll", _i), ("user_fn", _vp * 4),
-        ("max_distance", ctypes.c_double),
+        ("max_distance", ctypes.c_double), ("geo64", _vp), ("geo64_n", ctypes.c_longlong),
     ]
 
 
@@ -91,15 +92,41 @@ class PlanDesc(ctypes.Structure):
 _LIB = None
 
 
+def _ensure_current():
+    """Make sure the library file was built from the sources beside it (the hash of csrc/ and
+    include/ embedded at build time); a missing or stale library is rebuilt unless
+    ``SKELFMM_AMD_NO_REBUILD`` is set, in which case it is an error."""
+    from . import _build
+
+    want = _build.source_hash()
+    have = _build.built_hash(LIB_PATH)
+    if have == want:
+        return want
+    what = "is not built" if have is None else f"was built from other sources ({have}, sources {want})"
+    if os.environ.get("SKELFMM_AMD_NO_REBUILD"):
+        raise RuntimeError(f"HIP library {LIB_PATH} {what}; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    import sys
+
+    print(f"skelfmm_amd: HIP library {what}; compiling it for gfx950", file=sys.stderr, flush=True)
+    _build.build_library()
+    return want
+
+
+def build_id():
+    """Source hash of the loaded library (equal to that of the sources beside it)."""
+    return lib().skf_build_id().decode()
+
+
 def lib():
-    """Load the HIP library (raises ``RuntimeError`` if it is not built)."""
+    """Load the HIP library, compiling it first when it is missing or older than its sources."""
     global _LIB
     if _LIB is not None:
         return _LIB
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(
-            f"HIP library {LIB_PATH} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    want = _ensure_current()
     L = ctypes.CDLL(LIB_PATH)
+    L.skf_build_id.restype = ctypes.c_char_p
+    if L.skf_build_id().decode() != want:
+        raise RuntimeError(f"HIP library {LIB_PATH}: loaded build id {L.skf_build_id().decode()} != {want}")
     L.skf_last_error.restype = ctypes.c_char_p
     L.skf_version.restype = _i
     L.skf_fill_zero.argtypes = [_vp, ctypes.c_longlong, _i, _vp]
@@ -128,7 +155,7 @@ def lib():
     L.skf_jit_log.restype = ctypes.c_char_p
     L.skf_jit_unload.argtypes = [_vp]
     for name in EXPORTED_SYMBOLS:
-        if name not in ("skf_last_error", "skf_version", "skf_build_last_error", "skf_jit_log"):
+        if name not in ("skf_last_error", "skf_version", "skf_build_id", "skf_build_last_error", "skf_jit_log"):
             getattr(L, name).restype = _i
     _LIB = L
     return L

@@ -106,7 +106,7 @@ def _host_build_bytes(fmm):
     if fmm is None:
         return 0
     arrays = list(fmm.cand) + [fmm.bs_list, fmm.rank_list, fmm.tree_boxes, fmm.tree_lev_sep]
-    return int(fmm.nbytes_proj() + sum(a.nbytes for a in arrays))
+    return int(fmm.nbytes_proj + sum(a.nbytes for a in arrays))
 
 
 def _largest_rank(fmm):

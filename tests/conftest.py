@@ -16,13 +16,9 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def hip_lib():
-    """The in-tree HIP library, built on demand (no GPU needed to load it)."""
+    """The in-tree HIP library, compiled on demand when missing or stale (no GPU needed to load it)."""
     from skelfmm_amd import hip
 
-    if not os.path.exists(hip.LIB_PATH):
-        from skelfmm_amd import _build
-
-        _build.build_library()
     return hip.lib()
 
 

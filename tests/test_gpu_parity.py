@@ -396,9 +396,10 @@ def test_captured_graph_outlives_setters():
 def test_fp32_double_layer_surface_conditioning():
     """Double layer on the unit sphere with normals = positions: rn = (x - y).n = -r^2/2 is
     ill-conditioned in the coordinates (rounding only the points to FP32 costs ~4e-3 in an exact
-    sum). The box-centred FP32 form must do at least as well as the oracle's restatement of the
-    reference's single-precision form on the same build, and the FP32 matrix-core and VALU paths
-    (which round the box-centre shift at different places) must differ by far less than that error."""
+    sum). The precision form forms rn in FP64 from FP64 rows (absolute points, normals, n.x), so
+    it must beat the oracle's restatement of the reference's single-precision form on the same
+    build by far, and the FP32 matrix-core and VALU paths (same rn, box-centre shift rounded at
+    different places) must agree to FP32 rounding."""
     from skelfmm_amd.batched import SkelFMMBatched
 
     X = sphere_points(5000, 4)
@@ -414,8 +415,12 @@ def test_fp32_double_layer_surface_conditioning():
     e_ref = rel_l2(ref32.astype(np.float64), y64)
     e_gpu = rel_l2(y_mfma, y64)
     print(f"sphere DL FP32: oracle single-precision form {e_ref:.2e}, GPU precision form {e_gpu:.2e}", flush=True)
-    assert e_gpu <= e_ref
-    assert rel_l2(y_mfma, y_valu) < 0.02 * e_gpu
+    e_paths = rel_l2(y_mfma, y_valu)
+    print(f"  matrix-core vs VALU near field: {e_paths:.2e}", flush=True)
+    # what remains is the FP32 rounding of the box-centred distances (1/r^3 amplifies it on the
+    # closest pairs of this coarse 5,000-point surface)
+    assert e_gpu <= 0.01 * e_ref and e_gpu <= 2e-5
+    assert e_paths <= 2e-5
 
 
 def test_autotune_picks_setting_and_keeps_parity(tmp_path, monkeypatch):
