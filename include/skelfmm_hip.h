@@ -207,6 +207,20 @@ int skf_scatter_result(const void* uvec, void* y, const int* leaf_slot, const in
 /* near-field translations; q and u as in skf_p2p_args */
 int skf_p2p(const skf_p2p_args* a, void* stream);
 
+/*
+ * skf_p2p without any allocation by or for the library: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The FP32 symmetric near field's FP64 reverse-sum
+ * buffer (urev) comes from a file-scope device array of SKF_NOALLOC_MAX_UREV doubles (32 MiB)
+ * that each GPU holds for the life of the process; a->urev must be NULL and a->urev_len (the
+ * scalars needed, 2 * nslots * width in the apply) at most SKF_NOALLOC_MAX_UREV, else
+ * SKF_ERR_ARG. Calls issue no hipMalloc, hipFree or allocating library routine. The array is
+ * shared by every call on a GPU and zeroed by each call before use: calls that overlap on one
+ * GPU (other streams, other threads) race on it.
+ */
+#define SKF_NOALLOC_MAX_UREV (1LL << 22)
+int skf_p2p_noalloc(const skf_p2p_args* a, void* stream);
+
 /* one upward level: gather children's skeleton charges, then qskel = q[:r] + T q[r:bs] */
 int skf_up_level(const skf_level_args* L, const void* T, int complex_t, void* qvec, void* qskel, int width,
                  int precision, void* stream);

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1973,6 +1974,53 @@ extern "C" int skf_p2p(const skf_p2p_args* a, void* stream) {
   if (!a) return set_err(SKF_ERR_ARG, "null p2p args");
   if (!a->q || !a->u) return set_err(SKF_ERR_ARG, "null vectors");
   return run_p2p(a, a->q, a->u, static_cast<hipStream_t>(stream));
+}
+
+// ---------------------------------------------- near field without allocation
+// The FP32 symmetric near field needs an FP64 buffer for its reverse sums (urev, one scalar per
+// potential index). skf_p2p_noalloc takes it from this file-scope array instead of from the
+// caller: every GPU the library runs on holds its own copy for the life of the process. In host
+// code `scratch` names a host-side stand-in, so its device address is looked up once per device
+// (hipGetSymbolAddress with that device current) and only that address is used.
+static __device__ __align__(256) double scratch[SKF_NOALLOC_MAX_UREV];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_address(double** out) {
+  int dev = -1;
+  SKF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return set_err(SKF_ERR_ARG, "device %d beyond the scratch table", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(SKF_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return SKF_OK;
+}
+}  // namespace
+
+extern "C" int skf_p2p_noalloc(const skf_p2p_args* a, void* stream) {
+  if (!a) return set_err(SKF_ERR_ARG, "null p2p args");
+  if (!a->q || !a->u) return set_err(SKF_ERR_ARG, "null vectors");
+  if (a->urev) return set_err(SKF_ERR_ARG, "skf_p2p_noalloc takes no urev buffer (it uses the library's own)");
+  if (a->urev_len < 0 || a->urev_len > (long long)SKF_NOALLOC_MAX_UREV)
+    return set_err(SKF_ERR_ARG, "urev_len %lld outside [0, %lld]: the library's reverse-sum buffer holds %lld doubles",
+                   a->urev_len, (long long)SKF_NOALLOC_MAX_UREV, (long long)SKF_NOALLOC_MAX_UREV);
+  skf_p2p_args b = *a;
+  if (a->urev_len > 0) {
+    int rc = scratch_address(&b.urev);
+    if (rc) return rc;
+  }
+  // run_p2p zeroes the first urev_len scalars with a fill kernel before the symmetric kernel
+  // accumulates into them, so nothing left by an earlier call is read
+  return run_p2p(&b, a->q, a->u, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int skf_up_level(const skf_level_args* L, const void* T, int complex_t, void* qvec, void* qskel,

@@ -26,6 +26,7 @@ EXPORTED_SYMBOLS = (
     "skf_gather_charges",
     "skf_scatter_result",
     "skf_p2p",
+    "skf_p2p_noalloc",
     "skf_up_level",
     "skf_down_level",
     "skf_plan_create",
@@ -133,6 +134,7 @@ def lib():
     L.skf_gather_charges.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]
     L.skf_scatter_result.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _i, _vp]
     L.skf_p2p.argtypes = [ctypes.POINTER(P2PArgs), _vp]
+    L.skf_p2p_noalloc.argtypes = [ctypes.POINTER(P2PArgs), _vp]
     L.skf_up_level.argtypes = [ctypes.POINTER(LevelArgs), _vp, _i, _vp, _vp, _i, _i, _vp]
     L.skf_down_level.argtypes = [ctypes.POINTER(LevelArgs), _vp, _i, _vp, _vp, _i, _i, _vp]
     L.skf_plan_create.argtypes = [ctypes.POINTER(PlanDesc), ctypes.POINTER(_vp)]

@@ -71,3 +71,20 @@ def test_apply_without_gpu_fails_loudly():
 
     with pytest.raises(RuntimeError, match="GPU"):
         SkelFMM(np.random.default_rng(0).random((200, 2)), operators.LAPLACE_2D, leaf_size=50)
+
+
+def test_noalloc_near_field_rejects_workspace_arguments(hip_lib):
+    """skf_p2p_noalloc refuses a caller buffer and reverse-sum lengths beyond its own buffer
+    before any device work (no GPU needed)."""
+    a = hip.P2PArgs()
+    assert hip_lib.skf_p2p_noalloc(None, None) == 1
+    fake = ctypes.c_void_p(16)  # never dereferenced: the checks come first
+    a.q, a.u = fake, fake
+    a.urev = fake
+    assert hip_lib.skf_p2p_noalloc(ctypes.byref(a), None) == 1
+    assert b"no urev" in hip_lib.skf_last_error()
+    a.urev = None
+    for bad in (-1, (1 << 22) + 1):
+        a.urev_len = bad
+        assert hip_lib.skf_p2p_noalloc(ctypes.byref(a), None) == 1
+        assert b"outside" in hip_lib.skf_last_error()
