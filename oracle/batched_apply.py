@@ -136,11 +136,18 @@ class BatchedApplyCPU:
         self.inds_lev_offset = lev_off
         self.skel_leaves = torch.as_tensor(b.skel_leaves)
 
+        # pair_fraction (CPU-baseline sampling, not part of the reference): keep each pair of every
+        # list with that probability; pairs_kept / pairs_total is the fraction actually run
+        self.pairs_total = 0
+        self.pairs_kept = 0
+
         def tl(x):
             x = np.asarray(x, dtype=np.int64).reshape(-1, 2)
+            self.pairs_total += x.shape[0]
             if pair_fraction is not None and x.shape[0] > 0:
                 keep = np.random.RandomState(seed + 1).rand(x.shape[0]) < pair_fraction
                 x = x[keep]
+            self.pairs_kept += x.shape[0]
             return torch.as_tensor(x)
 
         self.leaf_uskel_from_qskel_list = tl(b.leaf_uskel_from_qskel_list)
