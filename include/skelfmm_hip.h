@@ -311,6 +311,48 @@ int skf_mlp_backward(const float* params, const float* X, const float* target, i
 int skf_adam_step(float* params, const float* grad, float* m1, float* m2, int np, float lr, float beta1, float beta2,
                   float eps, int step, void* stream);
 
+/* ------------------------------------------------------------- GPU tree and lists
+ * Reference: the adaptive level-restricted 2^d-tree the reference imports from simpletree
+ * (skelfmm/skel_fmm.py:150, BalancedTree; its contract as skelfmm/recursive_skel.py:54-68,142-166
+ * uses it), the colleague / coarse neighbour queries get_boxes_colleague_neigh_fullvec and
+ * get_boxes_coarse_neigh_fullvec, and the pair lists of get_parallel_lists
+ * (skelfmm/recursive_skel.py:622-712). Boxes are numbered level by level, Morton order inside a
+ * level; `codes` holds every box's Morton code (level-major, sorted inside a level) and `sep` the
+ * first box of each level (nlevels + 1 entries). Keys are 64-bit unsigned.
+ */
+const char* skf_tree_last_error(void);
+/* key of point i: Morton interleave of q_k = clip(floor(((x_k - lo_k) / L0) 2^depth)); idx[i] = i */
+int skf_tree_keys(const double* pts, int n, int ndim, const double* lo, double L0, int depth, void* keys, int* idx,
+                  void* stream);
+/* stable radix sort of (key, value) pairs over key bits [0, end_bit); temp = NULL stores the
+   workspace size in *temp_bytes */
+int skf_tree_sort_pairs(void* temp, unsigned long long* temp_bytes, const void* keys_in, void* keys_out,
+                        const int* vals_in, int* vals_out, int n, int end_bit, void* stream);
+/* children of nparent boxes (codes, point ranges in sorted-key order) at child_level: entry
+   p 2^ndim + k = code p_code[p] 2^ndim + k and its point range (empty when start == end) */
+int skf_tree_child_ranges(const void* keys, int ndim, int depth, int child_level, int nparent,
+                          const long long* p_code, const int* p_start, const int* p_end, long long* c_code,
+                          int* c_start, int* c_end, void* stream);
+/* mark[b] = 1 for every box b without children (split[b] == 0) that a box of `level` touches
+   through a same-size cell while sitting more than one level above it (2:1 restriction) */
+int skf_tree_balance_marks(int ndim, int level, long long nbox_level, const long long* codes, const long long* sep,
+                           const int* split, int* mark, void* stream);
+/* level[b], icoord[b][ndim], children[b][2^ndim] (-1 absent) and parent[b] (caller fills -1) */
+int skf_tree_box_tables(int ndim, int nlevels, long long nboxes, const long long* codes, const long long* sep,
+                        int* level, int* icoord, long long* children, long long* parent, void* stream);
+/* rows of boxes [b0, b1), 3^ndim wide, padded with -1: same-level touching boxes incl. self */
+int skf_tree_colleagues(int ndim, const long long* codes, const long long* sep, const int* level, long long b0,
+                        long long b1, long long* out, void* stream);
+/* rows of boxes [b0, b1): touching leaves on coarser levels, ascending */
+int skf_tree_coarse(int ndim, const long long* codes, const long long* sep, const int* level,
+                    const unsigned char* is_leaf, long long b0, long long b1, long long* out, void* stream);
+/* pair lists of boxes [b0, b1) from their colleague and coarse rows: fill = 0 writes counts
+   [3 per box: leaf pairs, tree pairs, coarse pairs]; fill = 1 writes (box, other) pairs at the
+   exclusive offsets (3 per box) */
+int skf_tree_pairs(int width, const long long* coll, const long long* coarse, const unsigned char* is_leaf,
+                   long long b0, long long b1, int fill, long long* counts, const long long* offsets,
+                   long long* leaf_pairs, long long* tree_pairs, long long* coarse_pairs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,7 +9,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = [os.path.join(HERE, "csrc", "skelfmm_hip.hip"), os.path.join(HERE, "csrc", "skelfmm_build.hip")]
+SRC = [os.path.join(HERE, "csrc", f) for f in ("skelfmm_hip.hip", "skelfmm_build.hip", "skelfmm_tree.hip")]
 OUT = os.path.join(HERE, "libskelfmm_hip.so")
 
 HIPCC_FLAGS = [
@@ -66,16 +66,39 @@ def built_hash(path=OUT):
 
 
 def build_library(verbose=False):
-    tmp = OUT + ".tmp"
+    """Compile every translation unit to an object in parallel, then link the shared library."""
+    import shutil
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+
     tag = f'-DSKF_SOURCE_HASH="{source_hash()}"'
-    cmd = [hipcc()] + HIPCC_FLAGS + [tag, "-I", os.path.join(ROOT, "include")] + SRC + ["-lhiprtc", "-o", tmp]
-    if verbose:
-        print(" ".join(cmd))
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed to build libskelfmm_hip.so")
-    os.replace(tmp, OUT)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    objdir = tempfile.mkdtemp(prefix="skelfmm_objs_")
+    try:
+        objs = [os.path.join(objdir, os.path.basename(src) + ".o") for src in SRC]
+
+        def compile_one(pair):
+            src, obj = pair
+            cmd = [hipcc()] + compile_flags + [tag, "-I", os.path.join(ROOT, "include"), "-c", src, "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            return subprocess.run(cmd, capture_output=True, text=True)
+
+        with ThreadPoolExecutor(max_workers=len(SRC)) as pool:
+            results = list(pool.map(compile_one, zip(SRC, objs)))
+        for src, res in zip(SRC, results):
+            if res.returncode != 0:
+                sys.stderr.write(res.stdout + res.stderr)
+                raise RuntimeError(f"hipcc failed to compile {os.path.basename(src)}")
+        tmp = OUT + ".tmp"
+        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-lhiprtc", "-o", tmp]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stdout + res.stderr)
+            raise RuntimeError("linking libskelfmm_hip.so failed")
+        os.replace(tmp, OUT)
+    finally:
+        shutil.rmtree(objdir, ignore_errors=True)
     return OUT
 
 

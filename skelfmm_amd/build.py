@@ -337,7 +337,10 @@ class RecursiveSkeletonization:
         tree = self.tree
         leaf_mask = tree.are_leaves(np.arange(self.nboxes))
         leaf_pairs, tree_pairs, coarse_pairs = [], [], []
-        for lev in range(self.root_level + 1, self.nlevels):
+        if hasattr(tree, "pair_lists"):  # GPU tree: the same three lists from its device rows
+            lp, tp, cp = tree.pair_lists(self.root_level)
+            leaf_pairs, tree_pairs, coarse_pairs = [lp], [tp], [cp]
+        for lev in range(self.root_level + 1, self.nlevels) if not hasattr(tree, "pair_lists") else ():
             boxes = tree.get_boxes_level(lev)
             coll = tree.get_boxes_colleague_neigh_fullvec(boxes)
             coarse = tree.get_boxes_coarse_neigh_fullvec(boxes)

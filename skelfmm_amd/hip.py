@@ -45,6 +45,15 @@ EXPORTED_SYMBOLS = (
     "skf_jit_compile",
     "skf_jit_log",
     "skf_jit_unload",
+    "skf_tree_last_error",
+    "skf_tree_keys",
+    "skf_tree_sort_pairs",
+    "skf_tree_child_ranges",
+    "skf_tree_balance_marks",
+    "skf_tree_box_tables",
+    "skf_tree_colleagues",
+    "skf_tree_coarse",
+    "skf_tree_pairs",
 )
 
 _vp = ctypes.c_void_p
@@ -156,8 +165,19 @@ def lib():
                                   _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(ctypes.c_longlong)]
     L.skf_jit_log.restype = ctypes.c_char_p
     L.skf_jit_unload.argtypes = [_vp]
+    L.skf_tree_last_error.restype = ctypes.c_char_p
+    _ll = ctypes.c_longlong
+    L.skf_tree_keys.argtypes = [_vp, _i, _i, _vp, ctypes.c_double, _i, _vp, _vp, _vp]
+    L.skf_tree_sort_pairs.argtypes = [_vp, ctypes.POINTER(ctypes.c_ulonglong), _vp, _vp, _vp, _vp, _i, _i, _vp]
+    L.skf_tree_child_ranges.argtypes = [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.skf_tree_balance_marks.argtypes = [_i, _i, _ll, _vp, _vp, _vp, _vp, _vp]
+    L.skf_tree_box_tables.argtypes = [_i, _i, _ll, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+    L.skf_tree_colleagues.argtypes = [_i, _vp, _vp, _vp, _ll, _ll, _vp, _vp]
+    L.skf_tree_coarse.argtypes = [_i, _vp, _vp, _vp, _vp, _ll, _ll, _vp, _vp]
+    L.skf_tree_pairs.argtypes = [_i, _vp, _vp, _vp, _ll, _ll, _i, _vp, _vp, _vp, _vp, _vp, _vp]
     for name in EXPORTED_SYMBOLS:
-        if name not in ("skf_last_error", "skf_version", "skf_build_id", "skf_build_last_error", "skf_jit_log"):
+        if name not in ("skf_last_error", "skf_version", "skf_build_id", "skf_build_last_error", "skf_jit_log",
+                        "skf_tree_last_error"):
             getattr(L, name).restype = _i
     _LIB = L
     return L
@@ -171,6 +191,11 @@ def check(rc):
 def check_build(rc):
     if rc != 0:
         raise RuntimeError("skelfmm_hip (build): " + lib().skf_build_last_error().decode(errors="replace"))
+
+
+def check_tree(rc):
+    if rc != 0:
+        raise RuntimeError("skelfmm_hip (tree): " + lib().skf_tree_last_error().decode(errors="replace"))
 
 
 def require_gpu():
