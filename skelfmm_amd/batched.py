@@ -101,7 +101,12 @@ class SkelFMMBatched(torch.nn.Module):
         self.max_bs_leaf = skel_fmm.max_bs_leaf
         self.rank_list = torch.as_tensor(skel_fmm.rank_list)
         self.bs_list = torch.as_tensor(skel_fmm.bs_list)
-        return self.upload_layout(build_layout(skel_fmm, max_sources=max_sources), use_single_precision, device)
+        nf_dev = None
+        if getattr(skel_fmm, "build_device", "cpu") == "gpu":  # a GPU build also forms its work tables there
+            hip.require_gpu()
+            nf_dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+        L = build_layout(skel_fmm, max_sources=max_sources, near_field_device=nf_dev)
+        return self.upload_layout(L, use_single_precision, device)
 
     def upload_layout(self, L, use_single_precision=False, device=None):
         """Copy a box layout to the GPU once; applies then reuse it."""

@@ -82,8 +82,11 @@ def _box_items(lengths, step):
     return np.stack([idx, first], axis=1).astype(np.int32).reshape(-1, 2)
 
 
-def build_layout(fmm, max_sources=16384):
-    """Flatten a ``RecursiveSkeletonization`` build into device-ready arrays."""
+def build_layout(fmm, max_sources=16384, near_field_device=None):
+    """Flatten a ``RecursiveSkeletonization`` build into device-ready arrays.
+
+    ``near_field_device`` (a torch device) forms the near-field work tables there
+    (``gpu_layout.near_field_tables``, identical to the host tables); None forms them here."""
     tree = fmm.tree
     nb = fmm.nboxes
     bs = fmm.bs_list.astype(np.int64)
@@ -116,12 +119,8 @@ def build_layout(fmm, max_sources=16384):
 
     # charge gather / result scatter over all leaf slots (every point exactly once)
     leaves = tree.get_leaves()
-    ls, lu = [], []
-    for b in leaves:
-        ls.append(np.arange(off[b], off[b] + bs[b]))
-        lu.append(fmm.cand[b])
-    L.leaf_slot = np.concatenate(ls).astype(np.int32)
-    L.leaf_user = np.concatenate(lu).astype(np.int32)
+    L.leaf_slot = (np.repeat(off[leaves], bs[leaves]) + _ranges(bs[leaves])).astype(np.int32)
+    L.leaf_user = np.concatenate([fmm.cand[b] for b in leaves]).astype(np.int32)
     assert L.leaf_slot.shape[0] == fmm.N and np.unique(L.leaf_user).shape[0] == fmm.N
 
     # interpolation matrices, one ragged array; matrices a GPU build left in device memory
@@ -133,16 +132,15 @@ def build_layout(fmm, max_sources=16384):
     # upward stages: deepest level first; the root level only gathers
     up = []
     for lev in range(fmm.nlevels - 1, fmm.root_level - 1, -1):
-        gboxes = fmm.gathered_boxes_level(lev)
-        gd, gs = [], []
-        for P in gboxes:
-            cs = fmm.child_src[P]
-            if cs is None or cs.shape[0] == 0:
-                continue
-            gd.append(np.arange(off[P], off[P] + bs[P]))
-            gs.append(off[cs[:, 0]] + cs[:, 1])
-        gather_dst = np.concatenate(gd).astype(np.int32) if gd else np.zeros(0, np.int32)
-        gather_src = np.concatenate(gs).astype(np.int32) if gs else np.zeros(0, np.int32)
+        gboxes = [P for P in fmm.gathered_boxes_level(lev)
+                  if fmm.child_src[P] is not None and fmm.child_src[P].shape[0] > 0]
+        if gboxes:
+            gb = np.asarray(gboxes, dtype=np.int64)
+            gather_dst = (np.repeat(off[gb], bs[gb]) + _ranges(bs[gb])).astype(np.int32)
+            cs = np.concatenate([fmm.child_src[P] for P in gboxes])
+            gather_src = (off[cs[:, 0]] + cs[:, 1]).astype(np.int32)
+        else:
+            gather_dst = gather_src = np.zeros(0, np.int32)
         sboxes = fmm.skel_boxes_level(lev)
         sboxes = sboxes[rk[sboxes] > 0]
         tabs = _level_tables(fmm, sboxes, off, toff)
@@ -157,7 +155,9 @@ def build_layout(fmm, max_sources=16384):
     maxj = int(rk.max()) + 1 if rk.size else 1
     if parents:
         cs_all = np.concatenate([fmm.child_src[P] for P in parents]).astype(np.int64)
-        pos_all = np.concatenate([off[P] + np.arange(fmm.child_src[P].shape[0]) for P in parents])
+        pa = np.asarray(parents, dtype=np.int64)
+        lens = np.array([fmm.child_src[P].shape[0] for P in parents], dtype=np.int64)
+        pos_all = np.repeat(off[pa], lens) + _ranges(lens)
         keys = cs_all[:, 0] * maxj + cs_all[:, 1]
         korder = np.argsort(keys, kind="stable")
         keys_sorted, pos_sorted = keys[korder], pos_all[korder]
@@ -179,9 +179,22 @@ def build_layout(fmm, max_sources=16384):
         down.append(LevelStage(lev, *tabs, gather_dst, gather_src, items))
     L.down = down
 
-    _near_field(L, fmm, off, bs, rk, max_sources)
+    if near_field_device is not None:
+        from .gpu_layout import near_field_tables
+
+        full, split = near_field_tables(fmm, off, bs, rk, L.nslots, max_sources, near_field_device)
+        _set_near_field(L, full, split)
+    else:
+        _near_field(L, fmm, off, bs, rk, max_sources)
     L.max_sources = int(max_sources)
     return L
+
+
+def _set_near_field(L, full, split):
+    for k in NearLists.FIELDS + ("items", "item_cost", "interactions"):
+        setattr(L, k, getattr(full, k))
+    L.split = split
+    L.evaluations = L.interactions if split is None else split[0].interactions + split[1].interactions
 
 
 def _gather_T(fmm, nb, rk):

@@ -15,8 +15,8 @@ from the objects it produced (``_report_storage``). The apply itself is the devi
 ``SkelFMMBatched`` (HIP kernels); this class only converts host arrays.
 
 Opt-in extensions (keyword-only): ``multi_rhs=True`` lets ``apply`` take ``(N, R)``
-right-hand sides; ``build_device="gpu"`` runs the interpolative decompositions as batched HIP
-kernels; ``rank_model`` (a ``learned_rank.RankMLP`` or ``"default"``) switches those to the
+right-hand sides; ``build_device="gpu"`` builds the tree, its lists and the work tables on the GPU and
+runs the interpolative decompositions as batched HIP kernels; ``rank_model`` (a ``learned_rank.RankMLP`` or ``"default"``) switches those to the
 learned fixed-rank sketched form with adaptive fallback; ``save(path)`` /
 ``SkelFMM.load(path)`` write and read the built operator (``skelfmm_amd.serialize``).
 """
@@ -176,7 +176,12 @@ class SkelFMM:
         return model
 
     def _make_tree(self):
-        self.tree = BalancedTree(self.points, leaf_size=self.leaf_size)
+        if self.build_device == "gpu":  # the same tree (bit for bit), built by HIP kernels
+            from .gpu_tree import GPUBalancedTree
+
+            self.tree = GPUBalancedTree(self.points, leaf_size=self.leaf_size)
+        else:
+            self.tree = BalancedTree(self.points, leaf_size=self.leaf_size)
         self.nlevels, self.nboxes, self.nleaves = self.tree.nlevels, self.tree.nboxes, self.tree.nleaves
 
     def _make_skeletons(self, npoints_max, p, verbose):

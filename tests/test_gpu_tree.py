@@ -117,3 +117,35 @@ def test_gpu_tree_rejects_bad_input():
         GPUBalancedTree(np.random.default_rng(0).random((10, 3)), leaf_size=0)
     with pytest.raises(ValueError, match="coincident"):
         GPUBalancedTree(np.zeros((40, 2)), leaf_size=8)  # more coincident points than a leaf holds
+
+
+@pytest.mark.parametrize("name", ["laplace_3d", "helmholtz_3d", "helmholtz_double_layer_3d", "laplace_2d"])
+def test_device_near_field_tables_equal_host_tables(name):
+    """The near-field work tables formed on the device (rows, symmetric split, groups, entries,
+    items heaviest first) equal the host tables entry for entry."""
+    from skelfmm_amd.layout import NearLists, build_layout
+
+    _gpu()
+    rng = np.random.default_rng(5)
+    if name == "laplace_2d":
+        X, op, kappa, g = annulus_points(20000, 3), ops.LAPLACE_2D, 0.0, None
+    elif name == "helmholtz_double_layer_3d":
+        X = sphere_points(20000, 2)
+        op, kappa, g = ops.HELMHOLTZ_DOUBLE_LAYER_3D, 6.0, X
+    else:
+        X = _mixture(40000, seed=7)
+        op, kappa, g = (ops.LAPLACE_3D, 0.0, None) if name == "laplace_3d" else (ops.HELMHOLTZ_3D, 8.0, None)
+    f = make_build(X, op, kappa=kappa, tol=1e-5, leaf_size=64, geometry=g)
+    for ms in (512, 16384):
+        host = build_layout(f, max_sources=ms)
+        dev = build_layout(f, max_sources=ms, near_field_device=torch.device("cuda", 0))
+        fields = NearLists.FIELDS + ("items", "item_cost")
+        for k in fields + ("interactions", "evaluations"):
+            a, b = getattr(dev, k), getattr(host, k)
+            assert np.array_equal(np.asarray(a), np.asarray(b)), k
+        assert (dev.split is None) == (host.split is None)
+        for P, Q in zip(dev.split or (), host.split or ()):
+            assert P.tpi == Q.tpi and P.interactions == Q.interactions
+            for k in fields:
+                assert np.array_equal(getattr(P, k), getattr(Q, k)), "split." + k
+    print(f"{name}: {host.items.shape[0]} items, split {host.split is not None}: device tables identical", flush=True)
