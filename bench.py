@@ -169,6 +169,18 @@ def cpu_baseline_validation(cores):
 
 
 # ----------------------------------------------------------------- shared helpers
+def _tuned_summary(t):
+    """The autotuner's choice without its per-candidate timings."""
+    if not t:
+        return t
+    out = {k: t[k] for k in ("max_sources", "sym_unroll", "mfma_min_channels", "near_field_ms", "cached", "pinned")
+           if k in t}
+    lv = t.get("levels")
+    if lv:
+        out["levels"] = {k: lv[k] for k in ("up", "down", "levels_default_ms", "levels_tuned_ms")}
+    return out
+
+
 def graph_time(d, x, steps, warmup):
     """(seconds per graph replay of the whole apply, result)."""
     import torch
@@ -281,7 +293,7 @@ def run_annuli(args, steps, warmup, with_cpu):
         "data": "synthetic: 32 curvy annuli, N_i log-uniform in [1e4, 5e5] (default_rng(2)), float32 N(0,1) charges",
         "config": {"workload": "batched_annuli_laplace_2d", "instances": int(len(ns)), "N_total": ntot,
                    "nrhs": width, "tol": tol, "leaf_size": leaf, "graph": True,
-                   "interactions_per_apply": int(d32.interactions), "autotune": tuned},
+                   "interactions_per_apply": int(d32.interactions), "autotune": _tuned_summary(tuned)},
         "fp64": {"value": round(ntot * width / t64 / 1e6, 3), "ms_per_step": round(t64 * 1e3, 4),
                  "max_rel_l2_err": max(errs64)},
         "fp32_max_rel_l2_err": max(errs32),
@@ -408,6 +420,12 @@ def run_helmholtz(args, workload, double_layer, steps, warmup, with_cpu, n=None)
             f"{builds['build_speedup_learned_over_adaptive']:.2f}x, stats {st}")
         f_use, d64 = f_l, d_l
     d32 = SkelFMMBatched(tree, op, kappa=kappa).build_from_copy(f_use, use_single_precision=True)
+    tuned = None
+    if not args.no_tune:
+        from skelfmm_amd import autotune
+
+        tuned = {"complex64": _tuned_summary(autotune.tune(d32, 2)), "complex128": _tuned_summary(autotune.tune(d64, 2))}
+        log(f"{name} autotune: {tuned}")
 
     q128 = torch.as_tensor(q, dtype=torch.complex128, device=dev)
     q64 = q128.to(torch.complex64)
@@ -436,7 +454,7 @@ def run_helmholtz(args, workload, double_layer, steps, warmup, with_cpu, n=None)
         "data": data + "; charges RandomState(1) randn + 1j randn",
         "config": {"workload": name, "N": N, "kappa": kappa, "tol": tol, "leaf_size": leaf, "nrhs": 1,
                    "nlevels": int(tree.nlevels), "root_level": int(f_use.root_level),
-                   "interactions_per_apply": int(d32.interactions), **extra},
+                   "interactions_per_apply": int(d32.interactions), "autotune": tuned, **extra},
         "complex128": {"value": round(N / t128 / 1e6, 3), "ms_per_step": round(t128 * 1e3, 4)},
         "rel_l2_err": errs,
         "builds": builds,
@@ -501,7 +519,15 @@ def run_cube(args):
         f"interactions/apply {d64.interactions:.3e}")
 
     tuned = None
-    if not args.no_tune:
+    if args.max_sources or args.sym_unroll:  # a pinned setting (profiling runs)
+        for d in (d64, d32):
+            if args.max_sources:
+                d.set_near_field_split(args.max_sources)
+            if args.sym_unroll:
+                d.set_sym_unroll(args.sym_unroll)
+        tuned = {k: {"max_sources": int(d.layout.max_sources), "sym_unroll": int(d.sym_unroll or 2), "pinned": True}
+                 for k, d in (("fp64", d64), ("fp32", d32))}
+    elif not args.no_tune:
         from skelfmm_amd import autotune
 
         tuned = {"fp64": autotune.tune(d64, 1), "fp32": autotune.tune(d32, 1)}
@@ -547,7 +573,13 @@ def run_cube(args):
     if not args.no_gpu_build:
         from skelfmm_amd.gpu_tree import GPUBalancedTree
 
-        GPUBalancedTree(X[:4096], leaf_size=args.leaf)
+        # first-call costs (rocPRIM, the device-table kernels) outside the timing: a 20k-point build
+        wt = GPUBalancedTree(X[:20000], leaf_size=args.leaf)
+        wf = RecursiveSkeletonization(wt, op, kappa=0.0, tol=args.tol)
+        wf.skel_tree(npoints_max=500, device="gpu")
+        wf.setup_lists()
+        SkelFMMBatched(wt, op).build_from_copy(wf, use_single_precision=True)
+        del wt, wf
         torch.cuda.synchronize()
         tg = time.perf_counter()
         gtree = GPUBalancedTree(X, leaf_size=args.leaf)
@@ -629,9 +661,7 @@ def run_cube(args):
         "config": {"workload": "laplace_3d_uniform_cube", "N": N, "tol": args.tol, "leaf_size": args.leaf,
                    "nrhs": 1, "root_level": int(fmm.root_level), "nlevels": int(tree.nlevels),
                    "interactions_per_apply": int(d64.interactions),
-                   "autotune": None if tuned is None else {
-                       k: {kk: v[kk] for kk in ("max_sources", "sym_unroll", "mfma_min_channels") if kk in v}
-                       for k, v in tuned.items()}},
+                   "autotune": None if tuned is None else {k: _tuned_summary(v) for k, v in tuned.items()}},
         "build_id": _hip.build_id(),
         "fp64": {"value": round(N / t64 / 1e6, 3), "ms_per_step": round(t64 * 1e3, 4), "rel_l2_err": e64,
                  "eager_ms_per_step": round(e64_t * 1e3, 4)},
@@ -669,6 +699,8 @@ def main():
     ap.add_argument("--instances", type=int, default=32)
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--no-tune", action="store_true", help="skip the near-field autotuner (on by default)")
+    ap.add_argument("--max-sources", type=int, default=0, help="headline workload: pin the source-run length")
+    ap.add_argument("--sym-unroll", type=int, default=0, help="headline workload: pin the symmetric unroll")
     ap.add_argument("--no-gpu-build", action="store_true",
                     help="laplace3d_cube: skip the second build with the IDs on the GPU")
     args = ap.parse_args()

@@ -166,6 +166,10 @@ typedef struct skf_level_args {
   const int* gather_src;
   int nitems;                 /* int2 {box index, first row (up) / column (down)} */
   const int* items;
+  int variant;                /* launch shape (autotuned per level and width; 0 = default):
+                                 one channel: up 0/1/2 = 2/1/4 vector loads per lane and step,
+                                 down 0/1/2 = 8/4/16 rows per step; >= 16 real FP32 channels:
+                                 0 = matrix cores, 1 = VALU GEMM; ignored otherwise */
 } skf_level_args;
 
 typedef struct skf_plan_desc {

@@ -12,6 +12,9 @@ near-field launch over
 * where the symmetric near-field kernel runs (one channel, single-layer kernel),
   the number of sources per step of its inner loop (1, 2 or 4: instruction-level
   parallelism against registers, i.e. waves per SIMD),
+* per tree level and direction, the launch shape of the interpolation stage
+  (``tune_levels``: vector loads in flight per lane for one channel, matrix cores or
+  VALU GEMM for >= 16 real FP32 channels), each stage timed alone,
 
 keeps the fastest and caches the choice per (device name, operator, precision,
 width, slots, interactions) in a JSON file (``$SKELFMM_AMD_TUNE_CACHE``, default
@@ -66,13 +69,48 @@ def apply_setting(dev, setting):
         dev.set_near_field_split(int(setting["max_sources"]))
 
 
-def tune(dev, width=1, reps=3, use_cache=True, splits=SPLITS):
-    """Pick and apply the fastest near-field setting for ``dev`` at ``width`` channels; returns it."""
+def level_variant_choices(dev, width):
+    """Launch variants of the interpolation stages at ``width`` channels: loads in flight for one
+    channel (3 shapes), matrix cores or VALU GEMM for >= 16 real FP32 channels, else one."""
+    nch = width // 2 if dev.complex_vectors else width
+    if nch == 1:
+        return (0, 1, 2)
+    if nch >= 16 and dev.precision == 1 and not dev.complex_t:
+        return (0, 1)
+    return (0,)
+
+
+def tune_levels(dev, width=1, reps=3):
+    """Per upward and downward stage, the fastest launch variant at ``width`` channels (each
+    stage timed alone with HIP events); applies it and returns the choice with the timings."""
+    choices = level_variant_choices(dev, width)
+    timed = {}
+    for v in choices:
+        dev.set_level_variants(width, up=[v] * len(dev.layout.up), down=[v] * len(dev.layout.down))
+        timed[v] = dev.time_levels(width, reps=reps)
+    best, default_ms, tuned_ms = {}, 0.0, 0.0
+    for name in ("up", "down"):
+        n = len(timed[choices[0]][name])
+        pick = [min(choices, key=lambda v: timed[v][name][i]) for i in range(n)]
+        best[name] = pick
+        default_ms += sum(timed[0][name])
+        tuned_ms += sum(timed[p][name][i] for i, p in enumerate(pick))
+    dev.set_level_variants(width, up=best["up"], down=best["down"])
+    return {"up": best["up"], "down": best["down"], "levels_default_ms": round(default_ms, 4),
+            "levels_tuned_ms": round(tuned_ms, 4),
+            "levels_ms": {str(v): {k: [round(t, 4) for t in ts] for k, ts in timed[v].items()} for v in choices}}
+
+
+def tune(dev, width=1, reps=3, use_cache=True, splits=SPLITS, levels=True):
+    """Pick and apply the fastest near-field setting for ``dev`` at ``width`` channels, and (with
+    ``levels``) the launch variant of every interpolation stage; returns the setting."""
     path = cache_path()
     key = signature(dev, width)
     db = _load(path) if use_cache else {}
     if key in db:
         apply_setting(dev, db[key])
+        if "levels" in db[key]:
+            dev.set_level_variants(width, up=db[key]["levels"]["up"], down=db[key]["levels"]["down"])
         return dict(db[key], cached=True)
     nch = width // 2 if dev.complex_vectors else width
     mfma_opts = [0]
@@ -94,6 +132,8 @@ def tune(dev, width=1, reps=3, use_cache=True, splits=SPLITS):
     best = {"max_sources": ms, "mfma_min_channels": mf, "sym_unroll": un, "near_field_ms": round(t, 4),
             "tried": [[round(a, 4), b, c, d] for a, b, c, d in results]}
     apply_setting(dev, best)
+    if levels and (len(dev.layout.up) or len(dev.layout.down)):
+        best["levels"] = tune_levels(dev, width, reps=reps)
     if use_cache:
         db[key] = best
         _store(path, db)

@@ -120,6 +120,7 @@ class SkelFMMBatched(torch.nn.Module):
         self.real_dtype = torch.float32 if use_single_precision else torch.float64
         self.complex_dtype = torch.complex64 if use_single_precision else torch.complex128
         self.mfma_min_channels = 0
+        self._level_variants = {}
         self.layout = L
         dev = self.device
         rd = self.real_dtype
@@ -309,17 +310,77 @@ class SkelFMMBatched(torch.nn.Module):
         d.T = hip.ptr(self.T)
         d.qbuf = hip.ptr(qbuf)
         d.ubuf = hip.ptr(ubuf)
+        up_arr, down_arr = self._level_arrays(width)
         d.nup = self._nup
-        d.up = ctypes.cast(self._up_arr, ctypes.POINTER(hip.LevelArgs))
+        d.up = ctypes.cast(up_arr, ctypes.POINTER(hip.LevelArgs))
         d.ndown = self._ndown
-        d.down = ctypes.cast(self._down_arr, ctypes.POINTER(hip.LevelArgs))
+        d.down = ctypes.cast(down_arr, ctypes.POINTER(hip.LevelArgs))
         urev = self._urev(width)
         d.p2p = self._p2p_args(width, urev)
         handle = ctypes.c_void_p()
         hip.check(hip.lib().skf_plan_create(ctypes.byref(d), ctypes.byref(handle)))
-        p = {"handle": handle, "qbuf": qbuf, "ubuf": ubuf, "urev": urev, "desc": d}
+        p = {"handle": handle, "qbuf": qbuf, "ubuf": ubuf, "urev": urev, "desc": d, "up_arr": up_arr,
+             "down_arr": down_arr}
         self._plans[width] = p
         return p
+
+    def _level_arrays(self, width):
+        """Per-width copies of the level stages with their launch variants (``set_level_variants``)."""
+        var = getattr(self, "_level_variants", {}).get(int(width), {})
+        arrays = []
+        for name, src, n in (("up", self._up_arr, self._nup), ("down", self._down_arr, self._ndown)):
+            arr = (hip.LevelArgs * max(1, n))()
+            vs = var.get(name, [])
+            for i in range(n):
+                arr[i] = hip.LevelArgs.from_buffer_copy(src[i])
+                arr[i].variant = int(vs[i]) if i < len(vs) else 0
+            arrays.append(arr)
+        return arrays
+
+    def set_level_variants(self, width, up=None, down=None):
+        """Launch shape of every upward / downward stage for ``width`` channels (see
+        ``skf_level_args.variant``; lists indexed like ``layout.up`` / ``layout.down``)."""
+        if not hasattr(self, "_level_variants"):
+            self._level_variants = {}
+        v = self._level_variants.setdefault(int(width), {})
+        if up is not None:
+            v["up"] = [int(x) for x in up]
+        if down is not None:
+            v["down"] = [int(x) for x in down]
+        self._invalidate()
+
+    def level_variants(self, width):
+        v = getattr(self, "_level_variants", {}).get(int(width), {})
+        return {"up": list(v.get("up", [0] * self._nup)), "down": list(v.get("down", [0] * self._ndown))}
+
+    def time_levels(self, width, reps=5):
+        """Mean milliseconds of every upward and downward stage for ``width`` channels (HIP events on
+        the current stream, the plan's own buffers): ({"up": [...], "down": [...]})."""
+        p = self._plan(width)
+        L = hip.lib()
+        st = hip.stream_handle(self.device)
+        es = p["qbuf"].element_size()
+        half = self.nslots * width * es
+        q0, u0 = p["qbuf"].data_ptr(), p["ubuf"].data_ptr()
+        cx = 1 if self.complex_t else 0
+        out = {}
+        for name, arr, n, fn, a, b in (("up", p["up_arr"], self._nup, L.skf_up_level, q0, q0 + half),
+                                       ("down", p["down_arr"], self._ndown, L.skf_down_level, u0, u0 + half)):
+            times = []
+            for i in range(n):
+                call = lambda: hip.check(fn(ctypes.byref(arr[i]), hip.ptr(self.T), cx, ctypes.c_void_p(a),
+                                            ctypes.c_void_p(b), width, self.precision, st))
+                call()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    call()
+                e1.record()
+                torch.cuda.synchronize(self.device)
+                times.append(e0.elapsed_time(e1) / reps)
+            out[name] = times
+        return out
 
     def _destroy_plan(self, width):
         p = self._plans.pop(width, None)

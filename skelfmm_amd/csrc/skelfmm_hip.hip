@@ -1456,7 +1456,7 @@ __global__ __launch_bounds__(256) void up_gemv_kernel(const R* __restrict__ T, c
 // One channel (one real or complex right-hand side): each wave takes its 4 rows of the item
 // together, so every load of the shared vector slice serves 4 rows and 5 loads are in flight per
 // step; one wave reduction per row at the end.
-template <typename R, bool CX>
+template <typename R, bool CX, int UNR = 2>
 __global__ __launch_bounds__(256) void up_gemv1_kernel(const R* __restrict__ T, const int* __restrict__ box_off,
                                                        const int* __restrict__ box_bs, const int* __restrict__ box_rank,
                                                        const long long* __restrict__ box_toff,
@@ -1479,7 +1479,7 @@ __global__ __launch_bounds__(256) void up_gemv1_kernel(const R* __restrict__ T, 
     Trow[rr] = T + (toff + (long long)min(j0 + rr, r - 1) * nred) * CWd;
   }
   const R* x = qvec + (long long)(off + r) * w;
-#pragma unroll 2
+#pragma unroll UNR
   for (int i = lane; i < nred; i += 64) {
     const R xr = x[(long long)i * w];
     const R xi = CX ? x[(long long)i * w + 1] : R(0);
@@ -1516,9 +1516,9 @@ __global__ __launch_bounds__(256) void up_gemv1_kernel(const R* __restrict__ T, 
   }
 }
 
-// One channel, downward: thread per column, rows unrolled by 8 (8 independent loads in flight);
+// One channel, downward: thread per column, rows unrolled by RB (RB independent loads in flight);
 // the skeleton potentials of a chunk of rows are staged in LDS and read as broadcasts.
-template <typename R, bool CX>
+template <typename R, bool CX, int RB = 8>
 __global__ __launch_bounds__(256) void down_gemv1_kernel(const R* __restrict__ T, const int* __restrict__ box_off,
                                                          const int* __restrict__ box_bs,
                                                          const int* __restrict__ box_rank,
@@ -1548,16 +1548,16 @@ __global__ __launch_bounds__(256) void down_gemv1_kernel(const R* __restrict__ T
     __syncthreads();
     if (act) {
       int jj = 0;
-      for (; jj + 8 <= nj; jj += 8) {
-        R tr[8], ti[8];
+      for (; jj + RB <= nj; jj += RB) {
+        R tr[RB], ti[RB];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < RB; ++k) {
           const R* tp = tcol + (long long)(j0 + jj + k) * nred * CWd;
           tr[k] = tp[0];
           ti[k] = CX ? tp[1] : R(0);
         }
 #pragma unroll
-        for (int k = 0; k < 8; k += 2) {
+        for (int k = 0; k < RB; k += 2) {
           cmac<R, CX>(ar0, ai0, tr[k], ti[k], us[(jj + k) * CWd], CX ? us[(jj + k) * CWd + 1] : R(0));
           cmac<R, CX>(ar1, ai1, tr[k + 1], ti[k + 1], us[(jj + k + 1) * CWd], CX ? us[(jj + k + 1) * CWd + 1] : R(0));
         }
@@ -1846,7 +1846,7 @@ int run_up_level(const skf_level_args* L, const void* T, int cx, void* qvec, voi
     if (nchan >= 16) {
       const dim3 g2(L->nitems, (nchan + 63) / 64);
       if constexpr (sizeof(R) == 4) {
-        if (!cx) {
+        if (!cx && L->variant != 1) {
           hipLaunchKernelGGL((interp_mfma_kernel<true>), g2, dim3(256), 0, st, static_cast<const float*>(T), L->box_off,
                              L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
           SKF_LAUNCHED();
@@ -1860,12 +1860,18 @@ int run_up_level(const skf_level_args* L, const void* T, int cx, void* qvec, voi
         hipLaunchKernelGGL((up_gemm_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
                            L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
     } else if (nchan == 1) {
-      if (cx)
-        hipLaunchKernelGGL((up_gemv1_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
-                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
-      else
-        hipLaunchKernelGGL((up_gemv1_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
-                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
+      // variant: vector loads in flight per lane and step (unroll 2 by default, 1, 4)
+      const dim3 g(L->nitems), blk(256);
+      const R* Tr = static_cast<const R*>(T);
+#define SKF_UP1(CXV, U) \
+  hipLaunchKernelGGL((up_gemv1_kernel<R, CXV, U>), g, blk, 0, st, Tr, L->box_off, L->box_bs, L->box_rank, L->box_toff, \
+                     items, qv, qs, w)
+      if (cx) {
+        if (L->variant == 1) SKF_UP1(true, 1); else if (L->variant == 2) SKF_UP1(true, 4); else SKF_UP1(true, 2);
+      } else {
+        if (L->variant == 1) SKF_UP1(false, 1); else if (L->variant == 2) SKF_UP1(false, 4); else SKF_UP1(false, 2);
+      }
+#undef SKF_UP1
     } else if (cx)
       hipLaunchKernelGGL((up_gemv_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
                          L->box_off, L->box_bs, L->box_rank, L->box_toff, items, qv, qs, w);
@@ -1890,7 +1896,7 @@ int run_down_level(const skf_level_args* L, const void* T, int cx, void* uvec, v
     const int nchan = cx ? w / 2 : w;
     if (nchan >= 16) {
       if constexpr (sizeof(R) == 4) {
-        if (!cx) {
+        if (!cx && L->variant != 1) {
           hipLaunchKernelGGL((interp_mfma_kernel<false>), dim3(L->nitems, (nchan + 63) / 64), dim3(256), 0, st,
                              static_cast<const float*>(T), L->box_off, L->box_bs, L->box_rank, L->box_toff, items, us,
                              uv, w);
@@ -1905,12 +1911,18 @@ int run_down_level(const skf_level_args* L, const void* T, int cx, void* uvec, v
         hipLaunchKernelGGL((down_gemm_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
                            L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
     } else if (nchan == 1) {
-      if (cx)
-        hipLaunchKernelGGL((down_gemv1_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
-                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
-      else
-        hipLaunchKernelGGL((down_gemv1_kernel<R, false>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
-                           L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
+      // variant: column loads in flight per thread and step (8 rows by default, 4, 16)
+      const dim3 g(L->nitems), blk(256);
+      const R* Tr = static_cast<const R*>(T);
+#define SKF_DN1(CXV, B) \
+  hipLaunchKernelGGL((down_gemv1_kernel<R, CXV, B>), g, blk, 0, st, Tr, L->box_off, L->box_bs, L->box_rank, \
+                     L->box_toff, items, uv, us, w)
+      if (cx) {
+        if (L->variant == 1) SKF_DN1(true, 4); else if (L->variant == 2) SKF_DN1(true, 16); else SKF_DN1(true, 8);
+      } else {
+        if (L->variant == 1) SKF_DN1(false, 4); else if (L->variant == 2) SKF_DN1(false, 16); else SKF_DN1(false, 8);
+      }
+#undef SKF_DN1
     } else if (cx)
       hipLaunchKernelGGL((down_gemv_kernel<R, true>), dim3(L->nitems), dim3(256), 0, st, static_cast<const R*>(T),
                          L->box_off, L->box_bs, L->box_rank, L->box_toff, items, uv, us, w);
@@ -1925,6 +1937,7 @@ int run_down_level(const skf_level_args* L, const void* T, int cx, void* uvec, v
 int check_level(const skf_level_args* L) {
   if (!L) return set_err(SKF_ERR_ARG, "null level");
   if (L->ngather < 0 || L->nitems < 0 || L->nboxes < 0) return set_err(SKF_ERR_ARG, "negative level sizes");
+  if (L->variant < 0 || L->variant > 2) return set_err(SKF_ERR_ARG, "level variant must be 0, 1 or 2");
   if (L->ngather > 0 && (!L->gather_dst || !L->gather_src)) return set_err(SKF_ERR_ARG, "missing gather maps");
   if (L->nitems > 0 && (!L->items || !L->box_off || !L->box_bs || !L->box_rank || !L->box_toff))
     return set_err(SKF_ERR_ARG, "missing box tables");

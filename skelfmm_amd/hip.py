@@ -86,7 +86,7 @@ class LevelArgs(ctypes.Structure):
     _fields_ = [
         ("nboxes", _i), ("box_off", _vp), ("box_bs", _vp), ("box_rank", _vp), ("box_toff", _vp),
         ("ngather", _i), ("gather_dst", _vp), ("gather_src", _vp),
-        ("nitems", _i), ("items", _vp),
+        ("nitems", _i), ("items", _vp), ("variant", _i),
     ]
 
 

@@ -477,3 +477,65 @@ def test_symmetric_near_field_matches_one_sided(name, single):
         ref = BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(q).numpy()
         if op is not ops.HELMHOLTZ_2D:
             assert rel_l2(y_sym, ref) <= FP64_TOL
+
+
+@pytest.mark.parametrize("single,nrhs", [(False, 1), (True, 1), (False, 1j), (True, 20)])
+def test_level_variants_keep_parity(single, nrhs):
+    """Every launch variant of the interpolation stages (loads in flight for one channel, matrix
+    cores or VALU GEMM at >= 16 FP32 channels) gives the same apply: FP64 within 1e-12 of the
+    oracle, FP32 variants within FP32 rounding of each other; the level autotuner keeps parity."""
+    from skelfmm_amd import autotune
+    from skelfmm_amd.batched import SkelFMMBatched
+
+    rng = np.random.default_rng(8)
+    if nrhs == 1j:  # complex vectors (Helmholtz), one channel
+        X, op, kappa, R = rng.random((8000, 3)), ops.HELMHOLTZ_3D, 4.0, 1
+    else:
+        X, op, kappa, R = rng.random((12000, 3)), ops.LAPLACE_3D, 0.0, int(nrhs)
+    fmm = make_build(X, op, kappa=kappa, tol=1e-6, leaf_size=150)
+    dev = SkelFMMBatched(fmm.tree, op, kappa=kappa).build_from_copy(fmm, use_single_precision=single)
+    q = rng.normal(size=(X.shape[0], R)) if R > 1 else rng.normal(size=X.shape[0])
+    if op.is_complex:
+        q = q + 1j * rng.normal(size=q.shape)
+    x = dev.prepare_input(torch.as_tensor(q).to(_gpu()))
+    width = (2 if dev.complex_vectors else 1) * R
+    choices = autotune.level_variant_choices(dev, width)
+    ys = {}
+    for v in choices:
+        dev.set_level_variants(width, up=[v] * len(dev.layout.up), down=[v] * len(dev.layout.down))
+        ys[v] = dev.apply_device(x).cpu().numpy()
+    print(f"single={single} width={width}: variants {choices}", flush=True)
+    if not single:
+        ref = BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(q).numpy()
+        for v, y in ys.items():
+            assert rel_l2(y, ref) <= FP64_TOL, v
+    for v, y in ys.items():
+        assert rel_l2(y, ys[0]) <= (1e-6 if single else 1e-14), v
+    res = autotune.tune_levels(dev, width, reps=1)
+    assert len(res["up"]) == len(dev.layout.up) and len(res["down"]) == len(dev.layout.down)
+    assert dev.level_variants(width) == {"up": res["up"], "down": res["down"]}
+    y = dev.apply_device(x).cpu().numpy()
+    assert rel_l2(y, ys[0]) <= (1e-6 if single else 1e-14)
+
+
+@pytest.mark.parametrize("op_name", ["laplace_3d", "laplace_2d"])
+def test_coincident_points_in_both_forms(op_name):
+    """Two distinct points at the same place, every other charge zero, one box (all direct): the
+    FP64 form follows the reference's clamp r^2 >= 1e-15 (util_batched.py:175-185); the FP32 form
+    evaluates K at r^2 = 1e-20 (its r^2 is accumulated on a 1e-20 bias: 1/r = 1e10 in 3D, 0.5 log
+    r^2 = -23.03 in 2D) rather than the reference FP32 clamp r^2 >= 1e-8. Self pairs stay zero."""
+    op = ops.LAPLACE_3D if op_name == "laplace_3d" else ops.LAPLACE_2D
+    d = 3 if op is ops.LAPLACE_3D else 2
+    X = np.random.default_rng(11).random((400, d))
+    X[7] = X[123]
+    q = np.zeros(X.shape[0])
+    q[7] = 2.0
+    fmm = make_build(X, op, tol=1e-6, leaf_size=500)
+    y64, _ = _device_apply(fmm, q)
+    y32, _ = _device_apply(fmm, q.astype(np.float32), single=True)
+    k64 = 1.0 / np.sqrt(1e-15) if d == 3 else 0.5 * np.log(1e-15)
+    k32 = 1.0 / np.sqrt(1e-20) if d == 3 else 0.5 * np.log(1e-20)
+    print(f"{op_name}: coincident pair FP64 {y64[123] / 2:.6e} FP32 {float(y32[123]) / 2:.6e}", flush=True)
+    assert abs(y64[123] / 2.0 - k64) <= 1e-12 * abs(k64)
+    assert abs(float(y32[123]) / 2.0 - k32) <= 1e-6 * abs(k32)
+    assert y64[7] == 0.0 and float(y32[7]) == 0.0  # the charged point's own self pair is zero
