@@ -870,32 +870,35 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   // Staging of the tile at stream position tbase (nsrc sources). Every LDS read that locates
   // rows (entry table walks) happens before the tile's LDS-direct loads are issued: a later LDS
   // read the compiler cannot tell apart from the loads in flight would wait for them.
-  // Charges: wave w loads 1 KiB chunks 4 w .. 4 w + 3 of the tile, chunk c holding rows
-  // (64 / CPR) c .. ; per-source data: one thread of four per row, into registers.
-  const int sr = tid >> 2;
-  const bool meta_thread = (tid & 3) == 0 && sr < TR;
+  // Charges: wave w loads 1 KiB chunks 4 w .. 4 w + 3 of the tile, which hold rows RW w ..
+  // RW w + RW - 1 (RW = TR / 4). Lane l of the wave locates row RW w + (l mod RW) once (one walk
+  // of the entry table per wave); a lane loading a chunk takes its row's charge offset from the
+  // lane that located it (a shuffle). Lanes l < RW also stage that row's per-source data.
+  constexpr int RW = TR / 4;      // rows per wave
+  constexpr int RPC = 64 / CPR;   // rows covered by one chunk instruction of a wave
+  static_assert(RW * 4 == TR && RPC * 4 == RW, "tile rows must split evenly over the waves");
+  const int sr = wave * RW + (lane % RW);
+  const bool meta_thread = lane < RW;
   R rp[PS], rsh[3];
   double rg[4] = {0.0, 0.0, 0.0, 0.0};
   int rself = -1;
   auto stage_issue = [&](R* qdst, int tbase, int nsrc) {
+    int s, j;
+    locate(tbase, sr < nsrc ? sr : 0, s, j);
+    const int qrow = (etab ? e_meta[s][0] : d.e_q[s]) + j;
     const R* src[4];
 #pragma unroll
     for (int ii = 0; ii < 4; ++ii) {
       const int slot = (wave * 4 + ii) * 64 + lane;  // 16-byte chunk of the tile
       const int row = slot / CPR, q = slot % CPR;
-      const int rr = row < nsrc ? row : 0;
-      int s, j;
-      locate(tbase, rr, s, j);
-      const int eq = etab ? e_meta[s][0] : d.e_q[s];
+      const int qr = __shfl(qrow, ii * RPC + lane / CPR, 64);
       const int col = min((q ^ ((row & 3) << QSH)) * EPC, cmax);
-      src[ii] = qv + (long long)(eq + j) * d.width + c0 + col;
+      src[ii] = qv + (long long)qr * d.width + c0 + col;
     }
     int sl = 0, fl = 0;
     R ox = 0, oy = 0, oz = 0;
     const bool have = meta_thread && sr < nsrc;
-    if (have) {
-      int s, j;
-      locate(tbase, sr, s, j);
+    if (have) {  // row sr: the entry walk above
       if (etab) {
         sl = e_meta[s][1] + j;
         fl = e_meta[s][2];
