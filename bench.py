@@ -52,11 +52,12 @@ F_OP = {"laplace_3d": 9, "laplace_2d": 9, "helmholtz_3d": 15, "helmholtz_double_
         "helmholtz_adjoint_double_layer_3d": 24}
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 HBM_BYTES_PER_S = 8.0e12
-# HBM bytes per launch of the FP32 symmetric near field (zero fill and fold included) on the
-# headline workload, from two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by the
-# same counters on a 512 MiB copy, keyed by (N, tol, leaf, symmetric unroll, source-run length)
-# of the profiled setting; profiles/near_field_laplace3d_uniform1M_hbm_traffic.md
-P2P_FP32_TRAFFIC_BYTES = {}
+# HBM bytes per launch of the FP32 symmetric kernel on the headline workload, from two
+# rocprofv3 --pmc passes (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00: the factors a 512 MiB copy in the
+# same run reads), keyed by (N, tol, leaf), with the setting that was profiled (symmetric unroll,
+# source-run length; the traffic barely depends on either); profiles/apply_laplace3d_uniform1M_hbm_traffic.md
+P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400): {"bytes": 584_339_456 + 635_866_112, "sym_unroll": 2,
+                                                   "max_sources": 4096}}
 
 
 def log(msg):
@@ -617,17 +618,19 @@ def run_cube(args):
     k_ms = s32 if sym else p32
     ach = served * fpi / k_ms / 1e12
     executed = (d32.sym_pairs * (fpi + 2) if sym else d32.interactions * fpi) / k_ms / 1e12
-    setting = (N, args.tol, args.leaf, int(d32.sym_unroll or 2), int(d32.layout.max_sources))
+    setting = (int(d32.sym_unroll or 2), int(d32.layout.max_sources))
+    profiled = P2P_FP32_TRAFFIC_BYTES.get((N, args.tol, args.leaf)) if sym else None
     roofline = {
         "bound": "valu",
         "achieved": round(ach, 3),
         "peak": PEAK_TFLOPS["fp32"],
         "unit": "TFLOP/s",
         "frac": round(ach / PEAK_TFLOPS["fp32"], 4),
-        "traffic": P2P_FP32_TRAFFIC_BYTES.get(setting) if sym else None,
-        "traffic_unit": "bytes per launch (HBM read + write) of the dominant kernel at the profiled setting "
-                        "(symmetric unroll, source-run length); null when the tuned setting was not profiled",
-        "traffic_setting": {"sym_unroll": setting[3], "max_sources": setting[4]},
+        "traffic": profiled["bytes"] if profiled else None,
+        "traffic_unit": "bytes per launch (HBM read + write) of the dominant kernel, rocprofv3 FETCH_SIZE / "
+                        "WRITE_SIZE scaled by a 512 MiB copy",
+        "traffic_profiled_setting": None if not profiled else {k: profiled[k] for k in ("sym_unroll", "max_sources")},
+        "run_setting": {"sym_unroll": setting[0], "max_sources": setting[1]},
         "kernel": ("p2p_sym_kernel<float, LAPLACE_3D> (pairs of lists, both directions; zero fill and fold of the "
                    "reverse sums included)" if sym else "p2p_kernel<float, LAPLACE_3D>"),
         "kernel_ms": round(k_ms * 1e3, 4),
