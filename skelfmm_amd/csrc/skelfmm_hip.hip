@@ -697,20 +697,34 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
 // two LDS buffers: the loads of tile i + 1 are issued before the matrix work of tile i and
 // waited for after it, so the staging latency overlaps the matrix work without holding the
 // charges in registers. A tile is TR sources x 64 channels (16 KiB: TR = 64 in FP32, 32 in
-// FP64), stored unpadded with channel c of row r at c ^ 16 (r & 3): the B-operand reads (rows
-// 4 kk .. 4 kk + 3, 16 channels each) fall on distinct banks. Staged charges are raw: the
-// entry sign multiplies the kernel value, and rows past the tile's sources load row 0 with
-// sign 0. The two buffers are distinct arrays and the tile loop is unrolled by two, so the
-// compiler can tell the LDS reads of the current tile from the loads in flight. Needs a width
-// that is a multiple of 16 bytes' worth of scalars and 16-byte aligned charges.
+// FP64), row-major. Column j of matrix block nb is channel 16 E (nb / E) + E j + nb mod E, E the
+// scalars per 16 bytes (4 in FP32, 2 in FP64), so the B operands a lane needs in a matrix step
+// come in 16-byte LDS reads (one in FP32, two in FP64) at offsets that are constants of the
+// unrolled loop, and the 16 lanes of a row read 256 contiguous bytes. Staged charges are raw:
+// the entry sign (times
+// the kernel's constant factor, 0.5 ln 2 for the FP32 2D Laplace kernel) multiplies the kernel
+// value, and rows past the tile's sources load row 0 with factor 0. The two buffers are distinct
+// arrays and the tile loop is unrolled by two, so the compiler can tell the LDS reads of the
+// current tile from the loads in flight. Needs a width that is a multiple of 16 bytes' worth of
+// scalars and 16-byte aligned charges.
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
 typedef __attribute__((address_space(1))) void* global_void_ptr;
 
 // vmcnt(0), other counters unconstrained (gfx9 s_waitcnt encoding)
 __device__ inline void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// lambdas of the kernels below are always inlined (a call would put their state on the stack)
+#define SKF_INLINE __attribute__((always_inline))
+
+template <int N>
+struct IntC {
+  static constexpr int value = N;
+};
+
+// FP32: four waves per SIMD (the LDS of four workgroups fits a CU)
 template <typename R, int OP, bool PREC>
-__global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1)))
+void p2p_mfma_dma_kernel(P2PDev d) {
   static_assert(OpT<OP>::cplx == 0, "real kernels only");
   constexpr bool G64 = PREC && Geo64<OP>::on;  // double layer, precision form: rn from FP64 rows
   using S = Stage<OP, 1, G64>;
@@ -718,20 +732,29 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   typedef typename MF::V V;
   constexpr int DIM = S::dim;
   constexpr int NG = S::ng;
-  constexpr int PS = NG ? 8 : 4;  // per-source row: x, y, z, sign (+ normals)
-  constexpr int SG = NG ? 6 : 3;  // position of the sign in the row
-  constexpr int TR = sizeof(R) == 4 ? 64 : 32;         // sources per tile
-  constexpr int EPC = 16 / int(sizeof(R));             // scalars per 16-byte chunk
-  constexpr int CPR = kMB / EPC;                       // chunks per tile row
-  constexpr int QSH = sizeof(R) == 4 ? 2 : 3;          // chunk-index shift of the row swizzle
-  __shared__ __attribute__((aligned(16))) R qa[TR * kMB];
-  __shared__ __attribute__((aligned(16))) R qb[TR * kMB];
-  __shared__ R pa[TR * PS];
-  __shared__ R pb[TR * PS];
+  // per-source row: x, y, z, factor (+ normals); the self index rides in a free slot of the row
+  // (z in 2D, the eighth value with normals), else in its own array
+  constexpr int PS = NG ? 8 : 4;
+  constexpr int SG = NG ? 6 : 3;                           // position of the factor in the row
+  constexpr int SELFP = DIM == 2 ? 2 : (NG ? 7 : -1);      // position of the self index, or -1
+  // FP32 2D Laplace: the kernel value is log2(r^2) times the staged factor (0.5 ln 2 folded in)
+  constexpr bool L2F = OP == SKF_LAPLACE_2D && sizeof(R) == 4;
+  constexpr R KSCALE = L2F ? R(0.34657359027997265) : R(1);
+  constexpr int TR = sizeof(R) == 4 ? 64 : 32;             // sources per tile
+  constexpr int EPC = 16 / int(sizeof(R));                 // scalars per 16-byte chunk
+  constexpr int CPR = kMB / EPC;                           // chunks per tile row
+  constexpr int RPI = 64 / CPR;                            // rows filled by one load instruction
+  constexpr int RW = TR / 4;                               // rows staged per wave
+  static_assert(4 * RPI == RW, "four load instructions cover a wave's rows");
+  typedef R QV __attribute__((ext_vector_type(EPC)));      // one 16-byte LDS read of B operands
+  __shared__ __attribute__((aligned(32))) R qa[TR * kMB];
+  __shared__ __attribute__((aligned(32))) R qb[TR * kMB];
+  __shared__ __attribute__((aligned(16))) R pa[TR * PS];
+  __shared__ __attribute__((aligned(16))) R pb[TR * PS];
   __shared__ double g64a[G64 ? TR * 4 : 1];  // FP64 normal-projection data per source (Geo64)
   __shared__ double g64b[G64 ? TR * 4 : 1];
-  __shared__ int sa[TR];
-  __shared__ int sbf[TR];
+  __shared__ int sa[SELFP < 0 ? TR : 1];
+  __shared__ int sbf[SELFP < 0 ? TR : 1];
   constexpr int kEnt = 128;
   __shared__ int e_start[kEnt + 1];
   __shared__ int e_meta[kEnt][3];  // charge offset, first point, flags
@@ -750,6 +773,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   const int tb = wave % TB, spart = wave / TB;
   const bool wave_busy = t0 + tb * 16 < ng;
   const int tloc = t0 + tb * 16 + (lane & 15);
+  const R tlocr = R(tloc);
   const bool tvalid = tloc < ng;
   const int tslot = d.g_pt[g] + (tvalid ? tloc : t0);
   const R* px = static_cast<const R*>(d.px);
@@ -773,8 +797,8 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
     cty = d.centers[3 * gb + 1];
     ctz = d.centers[3 * gb + 2];
   }
-  // FP32: matrix sums per tile, added into FP64 accumulators after every tile; FP64: the matrix
-  // accumulators themselves run across all tiles
+  // FP32: matrix sums over at most 64 sources per wave, then added into FP64 accumulators; FP64:
+  // the matrix accumulators themselves run across all tiles
   constexpr bool FOLD = sizeof(R) == 4;
   double acc[FOLD ? 4 : 1][4];
   V D[4];
@@ -786,6 +810,9 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[nb][r] = 0.0;
   }
+  // this lane's chunk of a staged row (clamped inside the row for narrow widths: channels past
+  // the last are never written out)
+  const int col = min((lane % CPR) * EPC, cmax);
 
   const int nent = se - sb;
   const bool etab = nent <= kEnt;  // uniform
@@ -839,7 +866,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   // row `off` of the tile at stream position tbase -> (entry, local index): through the LDS table
   // from the tile's first entry, or through the global lists from the tile's cursor
   int ce = 0, cs = sb, cj = 0;  // tile start: entry (LDS path); list position and offset (global path)
-  auto locate = [&](int tbase, int off, int& s, int& j) {
+  auto locate = [&](int tbase, int off, int& s, int& j) SKF_INLINE {
     if (etab) {
       const int pos = tbase + off;
       int e = ce;
@@ -856,7 +883,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
     }
   };
   // move the cursors to the next tile (TR positions on)
-  auto advance = [&](int next) {
+  auto advance = [&](int next) SKF_INLINE {
     if (etab) {
       while (ce < nent && e_start[ce + 1] <= next) ++ce;
     } else {
@@ -870,29 +897,23 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   // Staging of the tile at stream position tbase (nsrc sources). Every LDS read that locates
   // rows (entry table walks) happens before the tile's LDS-direct loads are issued: a later LDS
   // read the compiler cannot tell apart from the loads in flight would wait for them.
-  // Charges: wave w loads 1 KiB chunks 4 w .. 4 w + 3 of the tile, which hold rows RW w ..
-  // RW w + RW - 1 (RW = TR / 4). Lane l of the wave locates row RW w + (l mod RW) once (one walk
-  // of the entry table per wave); a lane loading a chunk takes its row's charge offset from the
-  // lane that located it (a shuffle). Lanes l < RW also stage that row's per-source data.
-  constexpr int RW = TR / 4;      // rows per wave
-  constexpr int RPC = 64 / CPR;   // rows covered by one chunk instruction of a wave
-  static_assert(RW * 4 == TR && RPC * 4 == RW, "tile rows must split evenly over the waves");
+  // Wave w stages rows RW w .. RW w + RW - 1: lane l < RW locates row RW w + l once (one walk of
+  // the entry table per wave) and stages its per-source data; load instruction ii of the wave
+  // fills rows RW w + RPI ii .. + RPI - 1, lane l loading chunk l mod CPR of row
+  // RW w + RPI ii + l / CPR at the charge offset that lane RPI ii + l / CPR located (a shuffle).
   const int sr = wave * RW + (lane % RW);
   const bool meta_thread = lane < RW;
   R rp[PS], rsh[3];
   double rg[4] = {0.0, 0.0, 0.0, 0.0};
   int rself = -1;
-  auto stage_issue = [&](R* qdst, int tbase, int nsrc) {
+  auto stage_issue = [&](R* qdst, int tbase, int nsrc) SKF_INLINE {
     int s, j;
     locate(tbase, sr < nsrc ? sr : 0, s, j);
     const int qrow = (etab ? e_meta[s][0] : d.e_q[s]) + j;
     const R* src[4];
 #pragma unroll
     for (int ii = 0; ii < 4; ++ii) {
-      const int slot = (wave * 4 + ii) * 64 + lane;  // 16-byte chunk of the tile
-      const int row = slot / CPR, q = slot % CPR;
-      const int qr = __shfl(qrow, ii * RPC + lane / CPR, 64);
-      const int col = min((q ^ ((row & 3) << QSH)) * EPC, cmax);
+      const int qr = __shfl(qrow, ii * RPI + lane / CPR, 64);
       src[ii] = qv + (long long)qr * d.width + c0 + col;
     }
     int sl = 0, fl = 0;
@@ -921,8 +942,8 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
     }
 #pragma unroll
     for (int ii = 0; ii < 4; ++ii)
-      __builtin_amdgcn_global_load_lds((global_void_ptr)src[ii], (lds_void_ptr)(qdst + (wave * 4 + ii) * 64 * EPC), 16,
-                                       0, 0);
+      __builtin_amdgcn_global_load_lds((global_void_ptr)src[ii], (lds_void_ptr)(qdst + (wave * RW + ii * RPI) * kMB),
+                                       16, 0, 0);
     // raw loads only: their values are used after the matrix work (meta_store), so no wait on
     // them (and on the loads issued before them) is needed here
     if (meta_thread) {
@@ -937,68 +958,130 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
           rp[4] = static_cast<const R*>(d.ny)[sl];
           rp[5] = static_cast<const R*>(d.nz)[sl];
         }
-        rp[SG] = (fl & SKF_SRC_NEG) ? R(-1) : R(1);
+        rp[SG] = (fl & SKF_SRC_NEG) ? -KSCALE : KSCALE;
         rsh[0] = ox;
         rsh[1] = oy;
         rsh[2] = oz;
         if constexpr (G64) Geo64<OP>::source(d.geo64, d.geo64_n, sl, rg);
       } else {
-        rp[0] = rp[1] = rp[2] = R(1e18);  // far away with sign 0: contributes exactly zero
+        rp[0] = rp[1] = rp[2] = R(1e18);  // far away with factor 0: contributes exactly zero
         rsh[0] = rsh[1] = rsh[2] = R(0);
         rg[0] = rg[1] = rg[2] = rg[3] = 0.0;
       }
     }
   };
-  auto meta_store = [&](R* __restrict__ pdst, int* __restrict__ sdst, double* __restrict__ gdst) {
+  auto meta_store = [&](R* __restrict__ pdst, int* __restrict__ sdst, double* __restrict__ gdst) SKF_INLINE {
     if (!meta_thread) return;
-    pdst[sr * PS + 0] = rp[0] - rsh[0];
-    pdst[sr * PS + 1] = rp[1] - rsh[1];
-    pdst[sr * PS + 2] = rp[2] - rsh[2];
+    R row[PS];
+    row[0] = rp[0] - rsh[0];
+    row[1] = rp[1] - rsh[1];
+    row[2] = rp[2] - rsh[2];
 #pragma unroll
-    for (int c = 3; c < PS; ++c) pdst[sr * PS + c] = rp[c];
+    for (int c = 3; c < PS; ++c) row[c] = rp[c];
+    // self index: exact in R (local indices stay far below 2^24)
+    if constexpr (SELFP >= 0) row[SELFP] = R(rself);
+#pragma unroll
+    for (int c = 0; c < PS; ++c) pdst[sr * PS + c] = row[c];
     if constexpr (G64) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) gdst[sr * 4 + c] = rg[c];
     }
-    sdst[sr] = rself;
+    if constexpr (SELFP < 0) sdst[sr] = rself;
   };
-  // matrix work of one staged tile
-  // (restrict parameters: the reads carry alias scopes, so they need not wait for the loads in flight)
-  auto compute = [&](const R* __restrict__ qcur, const R* __restrict__ pcur, const int* __restrict__ scur,
-                     const double* __restrict__ gcur, int nsrc) {
-    if (!wave_busy) return;
+  // FP32: add the matrix sums into the FP64 accumulators and restart them
+  auto fold = [&]() SKF_INLINE {
     if constexpr (FOLD) {
 #pragma unroll
-      for (int nb = 0; nb < 4; ++nb) D[nb] = V{0, 0, 0, 0};
-    }
-    const int ksteps = (nsrc + 3) >> 2;
-    const int k = lane >> 4;
-    for (int kk = spart; kk < ksteps; kk += SPL) {
-      const int sj = kk * 4 + k;
-      const R* row = &pcur[sj * PS];
-      const R dx = xt - row[0], dy = yt - row[1], dz = (DIM == 3) ? zt - row[2] : R(0);
-      R rn = 0;
-      if constexpr (G64) {
-        rn = R(Geo64<OP>::rn(t64, &gcur[sj * 4]));
-      } else {
-        if (OpT<OP>::geom == 1) rn = dx * row[3] + dy * row[4] + dz * row[5];
-        if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
-      }
-      R a = kernel_value<R, OP>(dx, dy, dz, rn, R(0)).re * row[SG];
-      if (scur[sj] == tloc) a = R(0);
-      const R* qrow = qcur + sj * kMB + (lane & 15);
-#pragma unroll
-      for (int nb = 0; nb < 4; ++nb) D[nb] = MF::mac(a, qrow[(nb ^ k) * 16], D[nb]);
-    }
-    if constexpr (FOLD) {
-#pragma unroll
-      for (int nb = 0; nb < 4; ++nb)
+      for (int nb = 0; nb < 4; ++nb) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[nb][r] += double(D[nb][r]);
+        D[nb] = V{0, 0, 0, 0};
+      }
     }
   };
-  // one tile: prefetch the next into (qn, pn, sn) while computing (qc, pc, sc)
-  auto step = [&](int base, R* qc, R* pc, int* sc, double* gc, R* qn, R* pn, int* sn, double* gn) {
+  int kacc = 0;  // matrix steps summed in D since the last fold (FP32)
+  // matrix work of one staged tile
+  // (restrict parameters: the reads carry alias scopes, so they need not wait for the loads in flight)
+  // (qcur2: channels 32 .. 63 of the tile for the second FP64 read, a pointer of its own so that
+  // this read too carries the scope that tells it apart from the loads in flight)
+  auto compute = [&](const R* __restrict__ qcur, const R* __restrict__ qcur2, const R* __restrict__ pcur,
+                     const int* __restrict__ scur, const double* __restrict__ gcur, int nsrc) SKF_INLINE {
+    if (!wave_busy) return;
+    const int ksteps = (nsrc + 3) >> 2;
+    // this wave's steps kk = spart, spart + SPL, ... (uniform)
+    const int nk = ksteps > spart ? (ksteps - spart + SPL - 1) / SPL : 0;
+    if constexpr (FOLD) {
+      if (kacc + nk > 16) {  // at most 16 steps (64 sources) per FP32 sum
+        fold();
+        kacc = 0;
+      }
+      kacc += nk;
+    }
+    const int k = lane >> 4;
+    const R* prow = pcur + (spart * 4 + k) * PS;
+    const R* qrow = qcur + (spart * 4 + k) * kMB + EPC * (lane & 15);
+    const R* qrow2 = qcur2 + (spart * 4 + k) * kMB + EPC * (lane & 15);
+    const int* srow = scur + (spart * 4 + k);
+    const double* grow = gcur + (spart * 4 + k) * 4;
+    // steps in groups of G, unrolled, at offsets that are constants: a group's extra steps read
+    // rows past the tile's sources (factor 0) and never rows past the tile
+    auto run = [&](auto spc) SKF_INLINE {
+      constexpr int SP = decltype(spc)::value;
+      constexpr int G = (TR / 4) / SP < 4 ? (TR / 4) / SP : 4;
+      for (int i = 0; i < nk; i += G) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+          const int o = (i + u) * SP * 4;  // row offset of this step
+          const R* row = prow + o * PS;
+          const R dx = xt - row[0], dy = yt - row[1], dz = (DIM == 3) ? zt - row[2] : R(0);
+          R kv;
+          if constexpr (L2F) {
+            kv = __builtin_amdgcn_logf(fma(dy, dy, fma(dx, dx, Mth<R>::clamp_cd)));
+          } else {
+            R rn = 0;
+            if constexpr (G64) {
+              rn = R(Geo64<OP>::rn(t64, grow + o * 4));
+            } else {
+              if (OpT<OP>::geom == 1) rn = dx * row[3] + dy * row[4] + dz * row[5];
+              if (OpT<OP>::geom == 2) rn = dx * gxt + dy * gyt + dz * gzt;
+            }
+            kv = kernel_value<R, OP>(dx, dy, dz, rn, R(0)).re;
+          }
+          R a = kv * row[SG];
+          if constexpr (SELFP >= 0) {
+            if (row[SELFP] == tlocr) a = R(0);
+          } else {
+            if (srow[o] == tloc) a = R(0);
+          }
+#pragma unroll
+          for (int rd = 0; rd < 4 / EPC; ++rd) {
+            const QV v = *reinterpret_cast<const QV*>((rd == 0 ? qrow : qrow2) + o * kMB);
+#pragma unroll
+            for (int e = 0; e < EPC; ++e) D[rd * EPC + e] = MF::mac(a, v[e], D[rd * EPC + e]);
+          }
+        }
+      }
+    };
+    if (SPL == 1)
+      run(IntC<1>{});
+    else if (SPL == 2)
+      run(IntC<2>{});
+    else
+      run(IntC<4>{});
+  };
+  // one tile: prefetch the next into one buffer while computing the other. The two buffer
+  // orders stay separate code (distinct markers): merged, the LDS reads would go through a
+  // pointer that may be either buffer and would wait for the loads in flight.
+  auto step = [&](auto bufc, int base) SKF_INLINE {
+    constexpr bool A = decltype(bufc)::value == 0;
+    R* qc = A ? qa : qb;
+    R* pc = A ? pa : pb;
+    int* sc = A ? sa : sbf;
+    double* gc = A ? g64a : g64b;
+    R* qn = A ? qb : qa;
+    R* pn = A ? pb : pa;
+    int* sn = A ? sbf : sa;
+    double* gn = A ? g64b : g64a;
     const int nsrc = min(TR, tot - base);
     const bool more = base + TR < tot;
     int nnext = 0;
@@ -1008,8 +1091,11 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
       stage_issue(qn, base + TR, nnext);
     }
     // keep the staging ahead of the matrix work (the compiler may otherwise sink it below)
-    asm volatile("" ::: "memory");
-    compute(qc, pc, sc, gc, nsrc);
+    if constexpr (A)
+      asm volatile("; tile in buffer a" ::: "memory");
+    else
+      asm volatile("; tile in buffer b" ::: "memory");
+    compute(qc, qc + 16 * EPC, pc, sc, gc, nsrc);
     if (more) meta_store(pn, sn, gn);
     wait_vm_loads();
     __syncthreads();
@@ -1024,20 +1110,32 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_dma_kernel(P2PDev d) {
   wait_vm_loads();
   __syncthreads();
   for (int base = 0; base < tot; base += 2 * TR) {
-    step(base, qa, pa, sa, g64a, qb, pb, sbf, g64b);
-    if (base + TR < tot) step(base + TR, qb, pb, sbf, g64b, qa, pa, sa, g64a);
+    step(IntC<0>{}, base);
+    if (base + TR < tot) step(IntC<1>{}, base + TR);
+  }
+  if (!wave_busy) return;
+  fold();
+  // The wave's 16 x 64 results go through LDS (every wave has passed the last barrier, so the
+  // tiles are no longer read; each wave takes a region of its own), so that each atomic
+  // instruction adds one target's 64 consecutive channels.
+  R* tr = sizeof(R) == 4 ? qa + wave * 16 * kMB : (wave < 2 ? qa : qb) + (wave & 1) * 16 * kMB;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    R* trow = tr + MF::row(lane, r) * kMB + EPC * (lane & 15);
+#pragma unroll
+    for (int rd = 0; rd < 4 / EPC; ++rd) {
+      QV v;
+#pragma unroll
+      for (int e = 0; e < EPC; ++e) v[e] = FOLD ? R(acc[FOLD ? rd * EPC + e : 0][r]) : R(D[rd * EPC + e][r]);
+      *reinterpret_cast<QV*>(trow + rd * 16 * EPC) = v;
+    }
   }
   R* uo = static_cast<R*>(d.u);
-#pragma unroll
-  for (int nb = 0; nb < 4; ++nb) {
-    const int ch = nb * 16 + (lane & 15);
-    if (ch >= ncv) continue;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int tl = t0 + tb * 16 + MF::row(lane, r);
-      if (wave_busy && tl < ng) atomic_acc(&uo[(long long)(d.g_out[g] + tl) * d.width + c0 + ch],
-                                          FOLD ? R(acc[FOLD ? nb : 0][r]) : R(D[nb][r]));
-    }
+  const int tb0 = t0 + tb * 16;
+#pragma unroll 4
+  for (int t = 0; t < 16; ++t) {
+    const R v = tr[t * kMB + lane];
+    if (tb0 + t < ng && lane < ncv) atomic_acc(&uo[(long long)(d.g_out[g] + tb0 + t) * d.width + c0 + lane], v);
   }
 }
 
