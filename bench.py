@@ -174,7 +174,8 @@ def _tuned_summary(t):
     """The autotuner's choice without its per-candidate timings."""
     if not t:
         return t
-    out = {k: t[k] for k in ("max_sources", "sym_unroll", "mfma_min_channels", "near_field_ms", "cached", "pinned")
+    out = {k: t[k] for k in ("max_sources", "sym_unroll", "mfma_min_channels", "item_order", "near_field_ms", "cached",
+                             "pinned")
            if k in t}
     lv = t.get("levels")
     if lv:

@@ -70,7 +70,9 @@ enum skf_status {
  *             u[out[g] + i] (i < n[g]); box id box[g] (precision form centre)
  *   entry s : points pt[s] .. pt[s]+n[s]-1 with charges q[qoff[s] + j];
  *             flags SKF_SRC_*; box id box[s]
- *   item    : int4 {group, first target, first entry, end entry}
+ *   item    : int4 {group, first target, first entry, end entry}; an item
+ *             with no entries (end entry <= first entry) is skipped, so a
+ *             launch order may pad the table with them
  *
  * Vectors hold `width` real scalars per slot ([slot][width]); for complex
  * kernels `width` = 2*nrhs (interleaved re/im). Real kernels treat complex

@@ -9,6 +9,8 @@ near-field launch over
 * for FP32 real kernels with at least 16 channels, the matrix-core kernel
   against the VALU kernel (RHS blocking: 64 channels per MFMA block, or the
   widest VALU channel block that fits LDS), and
+* the launch order of the work items (heaviest first, or spatial runs per XCD so that
+  neighbouring items share an L2),
 * where the symmetric near-field kernel runs (one channel, single-layer kernel),
   the number of sources per step of its inner loop (1, 2 or 4: instruction-level
   parallelism against registers, i.e. waves per SIMD),
@@ -28,6 +30,8 @@ import json
 import os
 
 import torch
+
+from .layout import ITEM_ORDERS
 
 SPLITS = (1024, 2048, 4096, 8192, 16384, 32768, 65536)
 
@@ -67,6 +71,9 @@ def apply_setting(dev, setting):
         dev.set_sym_unroll(int(setting.get("sym_unroll", 0)))
     if int(setting["max_sources"]) != int(getattr(dev.layout, "max_sources", -1)):
         dev.set_near_field_split(int(setting["max_sources"]))
+    order = setting.get("item_order", "heavy")
+    if hasattr(dev, "set_item_order") and order != getattr(dev, "item_order", "heavy"):
+        dev.set_item_order(order)
 
 
 def level_variant_choices(dev, width):
@@ -118,19 +125,23 @@ def tune(dev, width=1, reps=3, use_cache=True, splits=SPLITS, levels=True):
         mfma_opts = [0, -1]
     sym = getattr(dev, "use_symmetric", False) and nch == 1
     unrolls = (1, 2, 4) if sym else (0,)
+    orders = ITEM_ORDERS if hasattr(dev, "set_item_order") else ("heavy",)
     results = []
-    for mf in mfma_opts:
-        dev.set_mfma_min_channels(mf)
-        for un in unrolls:
-            if hasattr(dev, "set_sym_unroll"):
-                dev.set_sym_unroll(un)
-            for ms in splits:
-                dev.set_near_field_split(ms)
-                t = dev.time_near_field(width, reps=reps)
-                results.append((t, ms, mf, un))
-    t, ms, mf, un = min(results)
-    best = {"max_sources": ms, "mfma_min_channels": mf, "sym_unroll": un, "near_field_ms": round(t, 4),
-            "tried": [[round(a, 4), b, c, d] for a, b, c, d in results]}
+    for order in orders:
+        if hasattr(dev, "set_item_order"):
+            dev.set_item_order(order)
+        for mf in mfma_opts:
+            dev.set_mfma_min_channels(mf)
+            for un in unrolls:
+                if hasattr(dev, "set_sym_unroll"):
+                    dev.set_sym_unroll(un)
+                for ms in splits:
+                    dev.set_near_field_split(ms)
+                    t = dev.time_near_field(width, reps=reps)
+                    results.append((t, ms, mf, un, order))
+    t, ms, mf, un, order = min(results)
+    best = {"max_sources": ms, "mfma_min_channels": mf, "sym_unroll": un, "item_order": order,
+            "near_field_ms": round(t, 4), "tried": [[round(a, 4), b, c, d, e] for a, b, c, d, e in results]}
     apply_setting(dev, best)
     if levels and (len(dev.layout.up) or len(dev.layout.down)):
         best["levels"] = tune_levels(dev, width, reps=reps)

@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from . import hip, operators as operators_mod
-from .layout import build_layout, merge_layouts
+from .layout import ITEM_ORDERS, build_layout, merge_layouts, order_items
 
 __all__ = ["SkelFMMBatched", "BatchedInstances", "CapturedApply"]
 
@@ -190,7 +190,8 @@ class SkelFMMBatched(torch.nn.Module):
 
         self._nf = {k: i32(getattr(L, k)) for k in
                     ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags")}
-        self._nf["items"] = i32(L.items.reshape(-1))
+        self.item_order = "heavy"
+        self._nf["items"] = self._items_tensor(L)
         # symmetric split of the near field (single-layer kernels): pair lists + the rest
         self._split = None
         if getattr(L, "split", None) is not None:
@@ -198,7 +199,7 @@ class SkelFMMBatched(torch.nn.Module):
             for part in L.split:
                 t = {k: i32(getattr(part, k)) for k in
                      ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags")}
-                t["items"] = i32(part.items.reshape(-1))
+                t["items"] = self._items_tensor(part)
                 self._split.append(t)
             self._split_structs = [hip.P2PLists(), hip.P2PLists()]
             self._fill_split_structs()
@@ -438,12 +439,25 @@ class SkelFMMBatched(torch.nn.Module):
         from .layout import resplit
 
         resplit(self.layout, max_sources)
-        self._nf["items"] = torch.as_tensor(np.ascontiguousarray(self.layout.items.reshape(-1), dtype=np.int32),
-                                            device=self.device)
+        self._upload_items()
+
+    def set_item_order(self, mode):
+        """Launch order of the near-field work items: ``"heavy"`` (heaviest first, the default) or
+        ``"xcd"`` (spatial runs per XCD, so that neighbouring items share the XCD's L2)."""
+        if mode not in ITEM_ORDERS:
+            raise ValueError(f"item order must be one of {ITEM_ORDERS}, got {mode!r}")
+        self.item_order = mode
+        self._upload_items()
+
+    def _items_tensor(self, lists):
+        items, _ = order_items(lists.items, lists.item_cost, self.item_order)
+        return torch.as_tensor(np.ascontiguousarray(items.reshape(-1), dtype=np.int32), device=self.device)
+
+    def _upload_items(self):
+        self._nf["items"] = self._items_tensor(self.layout)
         if self._split is not None:
             for t, part in zip(self._split, self.layout.split):
-                t["items"] = torch.as_tensor(np.ascontiguousarray(part.items.reshape(-1), dtype=np.int32),
-                                             device=self.device)
+                t["items"] = self._items_tensor(part)
             self._fill_split_structs()
         self._invalidate()
 

@@ -359,6 +359,7 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
 
   const int4 it = d.items[blockIdx.x];
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
+  if (se <= sb) return;  // padding item of an XCD-ordered table (uniform: the whole workgroup)
   const int ch0 = blockIdx.y * NCH;
   const int nchv = min(NCH, d.nch - ch0);
   const int tid = threadIdx.x, lane = tid & 63;

@@ -317,6 +317,7 @@ __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
 
   const int4 it = d.items[blockIdx.x];
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
+  if (se <= sb) return;  // padding item of an XCD-ordered table (uniform: the whole workgroup)
   const int ng = d.g_n[g];
   double ctx = 0, cty = 0, ctz = 0;
   if (PREC) {
@@ -449,6 +450,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   __shared__ R e_shift[kEnt][3];   // precision form: centre difference into the target box frame
   const int4 it = d.items[blockIdx.x];
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
+  if (se <= sb) return;  // padding item of an XCD-ordered table (uniform: the whole workgroup)
   const int c0 = blockIdx.y * kMB;
   const int ncv = min(kMB, d.nch - c0);
   const int tid = threadIdx.x, lane = tid & 63;
@@ -761,6 +763,7 @@ void p2p_mfma_dma_kernel(P2PDev d) {
   __shared__ R e_shift[kEnt][3];
   const int4 it = d.items[blockIdx.x];
   const int g = it.x, t0 = it.y, sb = it.z, se = it.w;
+  if (se <= sb) return;  // padding item of an XCD-ordered table (uniform: the whole workgroup)
   const int c0 = blockIdx.y * kMB;
   const int ncv = min(kMB, d.nch - c0);
   const int cmax = d.width - c0 - EPC;  // last 16-byte chunk start inside the row (width % EPC == 0)
@@ -1004,12 +1007,16 @@ void p2p_mfma_dma_kernel(P2PDev d) {
   // (restrict parameters: the reads carry alias scopes, so they need not wait for the loads in flight)
   // (qcur2: channels 32 .. 63 of the tile for the second FP64 read, a pointer of its own so that
   // this read too carries the scope that tells it apart from the loads in flight)
-  auto compute = [&](const R* __restrict__ qcur, const R* __restrict__ qcur2, const R* __restrict__ pcur,
-                     const int* __restrict__ scur, const double* __restrict__ gcur, int nsrc) SKF_INLINE {
+  // (SP: source parts of the item, a template constant of the whole tile loop so that the matrix
+  // accumulators keep their registers from tile to tile)
+  auto compute = [&](auto spc, const R* __restrict__ qcur, const R* __restrict__ qcur2,
+                     const R* __restrict__ pcur, const int* __restrict__ scur, const double* __restrict__ gcur,
+                     int nsrc) SKF_INLINE {
+    constexpr int SP = decltype(spc)::value;
     if (!wave_busy) return;
     const int ksteps = (nsrc + 3) >> 2;
-    // this wave's steps kk = spart, spart + SPL, ... (uniform)
-    const int nk = ksteps > spart ? (ksteps - spart + SPL - 1) / SPL : 0;
+    // this wave's steps kk = spart, spart + SP, ... (uniform)
+    const int nk = ksteps > spart ? (ksteps - spart + SP - 1) / SP : 0;
     if constexpr (FOLD) {
       if (kacc + nk > 16) {  // at most 16 steps (64 sources) per FP32 sum
         fold();
@@ -1025,8 +1032,7 @@ void p2p_mfma_dma_kernel(P2PDev d) {
     const double* grow = gcur + (spart * 4 + k) * 4;
     // steps in groups of G, unrolled, at offsets that are constants: a group's extra steps read
     // rows past the tile's sources (factor 0) and never rows past the tile
-    auto run = [&](auto spc) SKF_INLINE {
-      constexpr int SP = decltype(spc)::value;
+    {
       constexpr int G = (TR / 4) / SP < 4 ? (TR / 4) / SP : 4;
       for (int i = 0; i < nk; i += G) {
 #pragma unroll
@@ -1061,18 +1067,12 @@ void p2p_mfma_dma_kernel(P2PDev d) {
           }
         }
       }
-    };
-    if (SPL == 1)
-      run(IntC<1>{});
-    else if (SPL == 2)
-      run(IntC<2>{});
-    else
-      run(IntC<4>{});
+    }
   };
   // one tile: prefetch the next into one buffer while computing the other. The two buffer
   // orders stay separate code (distinct markers): merged, the LDS reads would go through a
   // pointer that may be either buffer and would wait for the loads in flight.
-  auto step = [&](auto bufc, int base) SKF_INLINE {
+  auto step = [&](auto spc, auto bufc, int base) SKF_INLINE {
     constexpr bool A = decltype(bufc)::value == 0;
     R* qc = A ? qa : qb;
     R* pc = A ? pa : pb;
@@ -1095,7 +1095,7 @@ void p2p_mfma_dma_kernel(P2PDev d) {
       asm volatile("; tile in buffer a" ::: "memory");
     else
       asm volatile("; tile in buffer b" ::: "memory");
-    compute(qc, qc + 16 * EPC, pc, sc, gc, nsrc);
+    compute(spc, qc, qc + 16 * EPC, pc, sc, gc, nsrc);
     if (more) meta_store(pn, sn, gn);
     wait_vm_loads();
     __syncthreads();
@@ -1109,10 +1109,18 @@ void p2p_mfma_dma_kernel(P2PDev d) {
   // outstanding loads are the staging of the next tile
   wait_vm_loads();
   __syncthreads();
-  for (int base = 0; base < tot; base += 2 * TR) {
-    step(IntC<0>{}, base);
-    if (base + TR < tot) step(IntC<1>{}, base + TR);
-  }
+  auto tiles = [&](auto spc) SKF_INLINE {
+    for (int base = 0; base < tot; base += 2 * TR) {
+      step(spc, IntC<0>{}, base);
+      if (base + TR < tot) step(spc, IntC<1>{}, base + TR);
+    }
+  };
+  if (SPL == 1)
+    tiles(IntC<1>{});
+  else if (SPL == 2)
+    tiles(IntC<2>{});
+  else
+    tiles(IntC<4>{});
   if (!wave_busy) return;
   fold();
   // The wave's 16 x 64 results go through LDS (every wave has passed the last barrier, so the

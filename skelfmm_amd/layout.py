@@ -406,6 +406,56 @@ def make_items(g_n, g_e0, g_e1, e_n, max_sources, targets_per_item=TARGETS_PER_I
     return np.ascontiguousarray(items[order]), costs[order].astype(np.int64)
 
 
+ITEM_ORDERS = ("heavy", "xcd")
+# XCDs of the device: workgroup i of a launch runs on XCD i mod 8, each XCD with an L2 of its own
+N_XCD = 8
+
+
+def order_items(items, costs, mode="heavy", nxcd=N_XCD, chunks_per_xcd=4):
+    """Launch order of near-field work items (workgroup i takes item i).
+
+    ``"heavy"``: heaviest first (the stable cost order of ``make_items``), which shortens the
+    tail of the launch. ``"xcd"``: items in table order (group by group, i.e. box by box in
+    Morton order within a level, target blocks and source runs in turn) cut into
+    ``nxcd * chunks_per_xcd`` runs of equal cost; run c goes to XCD c mod nxcd and workgroup
+    ``x + nxcd p`` takes item p of XCD x's runs, so the items one XCD executes together are
+    spatial neighbours that read the same source sets through its L2. XCDs with fewer items are
+    padded with empty items ``{0, 0, 0, 0}`` (no source entries), which the kernels skip.
+    Returns (items, costs) as new arrays.
+    """
+    items = np.asarray(items, dtype=np.int32).reshape(-1, 4)
+    costs = np.asarray(costs, dtype=np.int64)
+    if mode not in ITEM_ORDERS:
+        raise ValueError(f"item order must be one of {ITEM_ORDERS}, got {mode!r}")
+    n = items.shape[0]
+    if mode == "heavy" or n == 0:
+        o = np.argsort(-costs, kind="stable")
+        return np.ascontiguousarray(items[o]), costs[o]
+    o = np.lexsort((items[:, 3], items[:, 2], items[:, 1], items[:, 0]))
+    items, costs = items[o], costs[o]
+    nrun = nxcd * chunks_per_xcd
+    cum = np.cumsum(costs)
+    total = max(int(cum[-1]), 1)
+    cut = np.searchsorted(cum, total * np.arange(1, nrun, dtype=np.float64) / nrun, side="left")
+    bounds = np.concatenate(([0], np.clip(cut, 0, n), [n]))
+    seqs = [np.concatenate([np.arange(bounds[c], bounds[c + 1]) for c in range(x, nrun, nxcd)])
+            for x in range(nxcd)]
+    L = max(s.shape[0] for s in seqs)
+    pos = np.full((L, nxcd), -1, dtype=np.int64)
+    for x, s in enumerate(seqs):
+        pos[:s.shape[0], x] = s
+    pos = pos.reshape(-1)
+    # drop trailing empty slots (nothing after them needs its XCD position)
+    last = np.nonzero(pos >= 0)[0][-1]
+    pos = pos[:last + 1]
+    out = np.zeros((pos.shape[0], 4), dtype=np.int32)
+    oc = np.zeros(pos.shape[0], dtype=np.int64)
+    real = pos >= 0
+    out[real] = items[pos[real]]
+    oc[real] = costs[pos[real]]
+    return out, oc
+
+
 def resplit(L, max_sources):
     """Rebuild a layout's near-field work items for another source-run length (in place)."""
     L.items, L.item_cost = make_items(L.g_n, L.g_e0, L.g_e1, L.e_n, max_sources)

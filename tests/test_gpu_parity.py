@@ -439,8 +439,46 @@ def test_autotune_picks_setting_and_keeps_parity(tmp_path, monkeypatch):
     ref = BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(q).numpy()
     assert rel_l2(y, ref) <= FP64_TOL
     assert best["sym_unroll"] in (1, 2, 4) and dev.sym_unroll == best["sym_unroll"]
+    assert best["item_order"] in ("heavy", "xcd") and dev.item_order == best["item_order"]
     again = autotune.tune(dev, 1, reps=1, splits=(2048, 16384))
     assert again["cached"] and again["max_sources"] == best["max_sources"]
+
+
+@pytest.mark.parametrize("name,single,nrhs", [("laplace_3d_cube", False, 1), ("laplace_3d_cube", True, 1),
+                                               ("laplace_2d_annulus", False, 20), ("laplace_2d_annulus", True, 20),
+                                               ("helmholtz_dl_sphere", False, 1), ("helmholtz_3d_cube", True, 1)])
+def test_item_orders_keep_parity(name, single, nrhs):
+    """The XCD launch order of the work items (spatial runs per XCD, padded with empty items) against
+    heaviest first: the same sums in another atomic order (FP64 within 1e-14, FP32 within 1e-6), and
+    FP64 within 1e-12 of the oracle; covers the symmetric, one-sided and matrix-core kernels."""
+    from skelfmm_amd.batched import SkelFMMBatched
+
+    X, op, kappa, geom, leaf = _case(name)
+    print(f"case {name} single={single} nrhs={nrhs}", flush=True)
+    fmm = make_build(X, op, kappa=kappa, tol=1e-6, leaf_size=leaf, geometry=geom)
+    dev = SkelFMMBatched(fmm.tree, op, kappa=kappa).build_from_copy(fmm, use_single_precision=single)
+    rs = np.random.RandomState(7)
+    q = rs.randn(X.shape[0], nrhs) if nrhs > 1 else rs.randn(X.shape[0])
+    if op.is_complex:
+        q = q + 1j * rs.randn(*q.shape)
+    if single and not op.is_complex:
+        q = q.astype(np.float32)
+    x = torch.as_tensor(q).to(_gpu())
+    y_heavy = dev.matvec(x).cpu().numpy()
+    for ms in (1024, 16384):
+        dev.set_near_field_split(ms)
+        dev.set_item_order("xcd")
+        assert dev.item_order == "xcd"
+        nitems = int(dev._nf["items"].shape[0] // 4)
+        assert nitems >= dev.layout.items.shape[0]  # padding items included
+        y_xcd = dev.matvec(x).cpu().numpy()
+        assert rel_l2(y_xcd, y_heavy) <= (1e-6 if single else 1e-14)
+        dev.set_item_order("heavy")
+    if not single:
+        ref = BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(q).numpy()
+        assert rel_l2(y_xcd, ref) <= FP64_TOL
+    with pytest.raises(ValueError):
+        dev.set_item_order("random")
 
 
 @pytest.mark.parametrize("name", ["laplace_2d_annulus", "laplace_3d_cube", "helmholtz_3d_cube", "helmholtz_2d_square"])

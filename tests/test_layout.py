@@ -7,7 +7,7 @@ from conftest import annulus_points, make_build, rel_l2, sphere_points
 from layout_emulator import emulate_apply
 from oracle.batched_apply import BatchedApplyCPU
 from skelfmm_amd import operators as ops
-from skelfmm_amd.layout import SRC_NEG, SRC_SELF, build_layout
+from skelfmm_amd.layout import N_XCD, SRC_NEG, SRC_SELF, build_layout, order_items
 
 
 def _builds():
@@ -265,3 +265,35 @@ def test_every_charge_slot_read_is_written_first(case):
     for P in lists:
         for q0, n in zip(P.e_q, P.e_n):
             assert written[q0:q0 + n].all(), "near field reads an unwritten charge slot"
+
+
+@pytest.mark.parametrize("case", BUILDS, ids=lambda c: c[0])
+def test_xcd_item_order_is_a_padded_permutation(case):
+    """The XCD launch order holds every item exactly once plus empty padding items; the items of
+    one XCD (workgroups x, x + 8, ...) are runs of the table order of equal cost."""
+    _, f, _ = case
+    L = build_layout(f, max_sources=700)
+    items, costs = order_items(L.items, L.item_cost, "xcd", chunks_per_xcd=2)
+    pad = items[:, 3] <= items[:, 2]
+    assert np.all(items[pad] == 0) and np.all(costs[pad] == 0)
+    real = items[~pad]
+    key = lambda a: sorted(map(tuple, a.tolist()))
+    assert key(real) == key(L.items)
+    assert np.array_equal(np.sort(costs[~pad]), np.sort(L.item_cost))
+    table = np.lexsort((L.items[:, 3], L.items[:, 2], L.items[:, 1], L.items[:, 0]))
+    rank = {tuple(L.items[i]): r for r, i in enumerate(table)}
+    per_xcd = []
+    for x in range(N_XCD):
+        mine = [rank[tuple(it)] for it in items[x::N_XCD] if it[3] > it[2]]
+        # in launch order each XCD walks its runs forward: positions rise within a run
+        assert sum(b < a for a, b in zip(mine, mine[1:])) <= 1
+        per_xcd.append(sum(int(c) for c, it in zip(costs[x::N_XCD], items[x::N_XCD]) if it[3] > it[2]))
+    # runs of equal cost up to one item each: two runs per XCD here
+    assert max(per_xcd) - min(per_xcd) <= 2 * 2 * int(L.item_cost.max())
+    # padding stops at the last real item: the table is as long as the busiest XCD needs
+    lmax = max(int((items[x::N_XCD, 3] > items[x::N_XCD, 2]).sum()) for x in range(N_XCD))
+    assert N_XCD * (lmax - 1) < items.shape[0] <= N_XCD * lmax and not pad[-1]
+    heavy, hc = order_items(L.items, L.item_cost, "heavy")
+    assert np.array_equal(heavy, L.items) and np.array_equal(hc, L.item_cost)
+    with pytest.raises(ValueError):
+        order_items(L.items, L.item_cost, "spiral")
