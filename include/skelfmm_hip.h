@@ -171,7 +171,9 @@ typedef struct skf_level_args {
   int variant;                /* launch shape (autotuned per level and width; 0 = default):
                                  one channel: up 0/1/2 = 2/1/4 vector loads per lane and step,
                                  down 0/1/2 = 8/4/16 rows per step; >= 16 real FP32 channels:
-                                 0 = matrix cores, 1 = VALU GEMM; ignored otherwise */
+                                 0 = matrix cores (the downward stage needs a width that is
+                                 a multiple of 4 and 16-byte aligned vectors, else VALU GEMM),
+                                 1 = VALU GEMM; ignored otherwise */
 } skf_level_args;
 
 typedef struct skf_plan_desc {

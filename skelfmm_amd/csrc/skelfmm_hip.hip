@@ -723,6 +723,10 @@ struct IntC {
   static constexpr int value = N;
 };
 
+// FP32 matrix sums run over at most this many steps (4 sources each) before they are added into
+// the FP64 accumulators
+constexpr int kFoldSteps = 32;
+
 // FP32: four waves per SIMD (the LDS of four workgroups fits a CU)
 template <typename R, int OP, bool PREC>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1)))
@@ -800,7 +804,7 @@ void p2p_mfma_dma_kernel(P2PDev d) {
     cty = d.centers[3 * gb + 1];
     ctz = d.centers[3 * gb + 2];
   }
-  // FP32: matrix sums over at most 64 sources per wave, then added into FP64 accumulators; FP64:
+  // FP32: matrix sums over at most 4 kFoldSteps sources per wave, then added into FP64 accumulators; FP64:
   // the matrix accumulators themselves run across all tiles
   constexpr bool FOLD = sizeof(R) == 4;
   double acc[FOLD ? 4 : 1][4];
@@ -1018,7 +1022,7 @@ void p2p_mfma_dma_kernel(P2PDev d) {
     // this wave's steps kk = spart, spart + SP, ... (uniform)
     const int nk = ksteps > spart ? (ksteps - spart + SP - 1) / SP : 0;
     if constexpr (FOLD) {
-      if (kacc + nk > 16) {  // at most 16 steps (64 sources) per FP32 sum
+      if (kacc + nk > kFoldSteps) {  // at most kFoldSteps steps per FP32 sum
         fold();
         kacc = 0;
       }
@@ -1375,6 +1379,19 @@ __global__ void fill_zero_kernel(R* p, long long n) {
   for (; i < n; i += stride) p[i] = R(0);
 }
 
+// zero fill with 16-byte stores (16-byte aligned base), scalars past the last whole vector
+template <typename R>
+__global__ void fill_zero16_kernel(R* p, long long n) {
+  typedef unsigned int U4 __attribute__((ext_vector_type(4)));
+  constexpr long long E = 16 / sizeof(R);
+  const long long nv = n / E;
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  U4* pv = reinterpret_cast<U4*>(p);
+  for (long long i = id; i < nv; i += stride) pv[i] = U4{0u, 0u, 0u, 0u};
+  for (long long i = nv * E + id; i < n; i += stride) p[i] = R(0);
+}
+
 // dst[ddst[i]*w + c] = src[dsrc[i]*w + c]  (or dst[ddst] with user/slot maps)
 // Row gathers dst[idst[i]][:] = src[isrc[i]][:] of w scalars, V scalars per thread (V divides
 // w, 16-byte accesses where the alignment allows) with 32-bit index arithmetic when the
@@ -1449,6 +1466,8 @@ __global__ void down_parent_kernel64(R* __restrict__ uvec, R* __restrict__ uskel
   }
 }
 
+// workgroups of a grid-stride launch of 256 threads (one pass over up to 2^28 elements)
+constexpr long long kMaxGrid = 1 << 20;
 int grid_for(long long n);
 
 // scalars per thread for rows of w scalars at these addresses: 16-byte vectors when w and the
@@ -1460,7 +1479,7 @@ int copy_width(int n, int w, const void* a, const void* b, const void* c = nullp
   if (w % int(16 / sizeof(R)) == 0 && al % 16 == 0) v = int(16 / sizeof(R));
   else if (sizeof(R) == 4 && w % 2 == 0 && al % 8 == 0) v = 2;
   const unsigned long long tot = (unsigned long long)n * (unsigned long long)(w / v);
-  if (tot + 65535ull * 256ull >= (1ull << 32)) return 0;
+  if (tot + (unsigned long long)kMaxGrid * 256ull >= (1ull << 32)) return 0;
   return v;
 }
 
@@ -1743,7 +1762,7 @@ __global__ __launch_bounds__(256) void down_gemv_kernel(const R* __restrict__ T,
 int grid_for(long long n) {
   long long g = (n + 255) / 256;
   if (g < 1) g = 1;
-  if (g > 65535) g = 65535;
+  if (g > kMaxGrid) g = kMaxGrid;
   return int(g);
 }
 
@@ -1875,13 +1894,13 @@ __global__ __launch_bounds__(256) void interp_mfma_kernel(const float* __restric
   const int nred = bs - r;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int cb = blockIdx.y * 64 + wave * 16;  // first channel of the wave's block
-  if (cb >= w) return;
   const int m = lane & 15, k = lane >> 4;
-  const int ch = cb + m;
-  const bool cv = ch < w;
   const float* Tb = T + toff;
   if (UP) {
+    const int cb = blockIdx.y * 64 + wave * 16;  // first channel of the wave's block
+    if (cb >= w) return;
+    const int ch = cb + m;
+    const bool cv = ch < w;
     // q_skel[j] = q[j] + sum_i T[j][i] q[r + i] for the item's rows j0 .. j0 + 15
     const int j0 = it.y;
     const bool rv = j0 + m < r;
@@ -1911,30 +1930,45 @@ __global__ __launch_bounds__(256) void interp_mfma_kernel(const float* __restric
       }
     }
   } else {
-    // u[r + i] += sum_j T[j][i] u_skel[j] for the item's columns, 16 at a time
+    // u[r + i] += sum_j T[j][i] u_skel[j]: wave w takes the item's 16-column blocks w, w + 4, ...
+    // with all 64 channels of the launch block as four matrix blocks that share the T operand.
+    // Column n of matrix block c is channel c0 + 4 n + c, so a lane's four B operands of a step
+    // are four consecutive channels (one 16-byte load; widths are multiples of 4 here) and its
+    // results of one row are four consecutive channels too.
     const int ibeg = it.y, iend = min(nred, it.y + kColsPerItem);
-    const float* xb = src + (long long)off * w + (cv ? ch : cb);
-    for (int ib = ibeg; ib < iend; ib += 16) {
+    const int c4 = blockIdx.y * 64 + 4 * m;  // this lane's first channel
+    const bool cv = c4 < w;
+    const float* xb = src + (long long)off * w + (cv ? c4 : 0);
+    for (int ib = ibeg + 16 * wave; ib < iend; ib += 64) {
       const int ic = ib + m;
       const bool iv = ic < iend;
-      v4f D = v4f{0, 0, 0, 0};
+      v4f D[4] = {v4f{0, 0, 0, 0}, v4f{0, 0, 0, 0}, v4f{0, 0, 0, 0}, v4f{0, 0, 0, 0}};
       for (int j0 = 0; j0 < r; j0 += 4 * kInterpSteps) {
-        float a[kInterpSteps], x[kInterpSteps];
+        float a[kInterpSteps];
+        v4f x[kInterpSteps];
 #pragma unroll
         for (int u = 0; u < kInterpSteps; ++u) {
           const int j = j0 + 4 * u + k;
           const bool jv = j < r;
           a[u] = (iv && jv) ? Tb[(long long)j * nred + ic] : 0.0f;
-          x[u] = (jv && cv) ? xb[(long long)j * w] : 0.0f;
+          x[u] = (jv && cv) ? *reinterpret_cast<const v4f*>(xb + (long long)j * w) : v4f{0, 0, 0, 0};
         }
 #pragma unroll
-        for (int u = 0; u < kInterpSteps; ++u) D = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], x[u], D, 0, 0, 0);
+        for (int u = 0; u < kInterpSteps; ++u)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) D[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u], x[u][c], D[c], 0, 0, 0);
       }
       if (cv) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int i = ib + 4 * k + q;
-          if (i < iend) dst[(long long)(off + r + i) * w + ch] += D[q];
+          if (i < iend) {
+            v4f* o = reinterpret_cast<v4f*>(dst + (long long)(off + r + i) * w + c4);
+            v4f v = *o;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] += D[c][q];
+            *o = v;
+          }
         }
       }
     }
@@ -2005,7 +2039,10 @@ int run_down_level(const skf_level_args* L, const void* T, int cx, void* uvec, v
     const int nchan = cx ? w / 2 : w;
     if (nchan >= 16) {
       if constexpr (sizeof(R) == 4) {
-        if (!cx && L->variant != 1) {
+        // 16-byte vector accesses: a width that is a multiple of 4 and aligned vectors
+        const bool v4ok = w % 4 == 0 && (reinterpret_cast<unsigned long long>(uv) & 15ull) == 0 &&
+                          (reinterpret_cast<unsigned long long>(us) & 15ull) == 0;
+        if (!cx && L->variant != 1 && v4ok) {
           hipLaunchKernelGGL((interp_mfma_kernel<false>), dim3(L->nitems, (nchan + 63) / 64), dim3(256), 0, st,
                              static_cast<const float*>(T), L->box_off, L->box_bs, L->box_rank, L->box_toff, items, us,
                              uv, w);
@@ -2060,10 +2097,18 @@ extern "C" int skf_fill_zero(void* ptr, long long n, int precision, void* stream
   if (n <= 0) return SKF_OK;
   if (!ptr) return set_err(SKF_ERR_ARG, "null pointer");
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (precision == 0)
+  const size_t rs = precision == 0 ? sizeof(double) : sizeof(float);
+  if ((reinterpret_cast<unsigned long long>(ptr) & 15ull) == 0) {
+    const dim3 g(grid_for((n * (long long)rs + 15) / 16));
+    if (precision == 0)
+      hipLaunchKernelGGL((fill_zero16_kernel<double>), g, dim3(256), 0, st, static_cast<double*>(ptr), n);
+    else
+      hipLaunchKernelGGL((fill_zero16_kernel<float>), g, dim3(256), 0, st, static_cast<float*>(ptr), n);
+  } else if (precision == 0) {
     hipLaunchKernelGGL((fill_zero_kernel<double>), dim3(grid_for(n)), dim3(256), 0, st, static_cast<double*>(ptr), n);
-  else
+  } else {
     hipLaunchKernelGGL((fill_zero_kernel<float>), dim3(grid_for(n)), dim3(256), 0, st, static_cast<float*>(ptr), n);
+  }
   SKF_LAUNCHED();
   return SKF_OK;
 }
