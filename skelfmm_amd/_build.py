@@ -11,6 +11,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = [os.path.join(HERE, "csrc", f) for f in ("skelfmm_hip.hip", "skelfmm_build.hip", "skelfmm_tree.hip")]
 OUT = os.path.join(HERE, "libskelfmm_hip.so")
+# the PyTorch operator library (host C++ over the C ABI, linked against OUT)
+TORCH_SRC = os.path.join(HERE, "csrc", "torch_ops.cpp")
+TORCH_OUT = os.path.join(HERE, "libskelfmm_torch.so")
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -37,7 +40,7 @@ BUILD_TAG = b"skelfmm-build-id:"
 def source_files():
     """Every file the library is compiled from: the HIP sources, their headers, the C ABI header."""
     files = [os.path.join(HERE, "csrc", f) for f in os.listdir(os.path.join(HERE, "csrc"))
-             if f.endswith((".hip", ".h"))]
+             if f.endswith((".hip", ".h", ".cpp"))]
     files += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")]
     return sorted(files)
 
@@ -91,7 +94,8 @@ def build_library(verbose=False):
                 sys.stderr.write(res.stdout + res.stderr)
                 raise RuntimeError(f"hipcc failed to compile {os.path.basename(src)}")
         tmp = OUT + ".tmp"
-        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-lhiprtc", "-o", tmp]
+        cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-soname,libskelfmm_hip.so"] + objs + \
+            ["-lhiprtc", "-o", tmp]
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
             sys.stderr.write(res.stdout + res.stderr)
@@ -102,5 +106,33 @@ def build_library(verbose=False):
     return OUT
 
 
+def build_torch_ops(verbose=False):
+    """Compile the PyTorch operator library (``torch.ops.skelfmm.apply``) with the host compiler
+    against PyTorch's headers and libraries; it links the HIP library beside it (rpath $ORIGIN)
+    and carries the same source hash."""
+    import torch
+    from torch.utils import cpp_extension
+
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    inc = cpp_extension.include_paths()
+    libdir = cpp_extension.library_paths()[0]
+    tmp = TORCH_OUT + ".tmp"
+    cmd = (["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-D_GLIBCXX_USE_CXX11_ABI={int(torch._C._GLIBCXX_USE_CXX11_ABI)}",
+            "-DUSE_ROCM", "-D__HIP_PLATFORM_AMD__", f'-DSKF_SOURCE_HASH="{source_hash()}"']
+           + [f"-I{p}" for p in inc] + [f"-I{rocm}/include", "-I", os.path.join(ROOT, "include"), TORCH_SRC,
+                                        f"-L{libdir}", "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_hip",
+                                        f"-L{HERE}", "-lskelfmm_hip", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{libdir}",
+                                        "-o", tmp])
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("compiling libskelfmm_torch.so failed")
+    os.replace(tmp, TORCH_OUT)
+    return TORCH_OUT
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_torch_ops(verbose=True))

@@ -422,7 +422,8 @@ class SkelFMMBatched(torch.nn.Module):
         if out is None:
             out = torch.empty_like(q)
         yr = torch.view_as_real(out) if out.is_complex() else out
-        hip.check(hip.lib().skf_apply(p["handle"], hip.ptr(xr), hip.ptr(yr), hip.stream_handle(self.device)))
+        # the registered PyTorch operator (C++ extension over skf_apply), on the current stream
+        hip.torch_ops().apply(p["handle"].value, xr, yr)
         return out
 
     def matvec(self, q, verbose=False):

@@ -13,10 +13,11 @@ import os
 
 import torch
 
-__all__ = ["lib", "check", "stream_handle", "P2PArgs", "P2PLists", "LevelArgs", "PlanDesc", "p2p_blocks", "LIB_PATH",
+__all__ = ["lib", "torch_ops", "check", "stream_handle", "P2PArgs", "P2PLists", "LevelArgs", "PlanDesc", "p2p_blocks", "LIB_PATH",
            "EXPORTED_SYMBOLS"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libskelfmm_hip.so")
+TORCH_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libskelfmm_torch.so")
 
 EXPORTED_SYMBOLS = (
     "skf_last_error",
@@ -120,6 +121,32 @@ def _ensure_current():
     print(f"skelfmm_amd: HIP library {what}; compiling it for gfx950", file=sys.stderr, flush=True)
     _build.build_library()
     return want
+
+
+_TORCH_OPS = None
+
+
+def torch_ops():
+    """The registered PyTorch operators ``torch.ops.skelfmm`` (``apply(plan, x, y)``: the whole
+    apply of a plan on the current stream), from ``libskelfmm_torch.so``, a thin C++ extension
+    over the C ABI built with the HIP library; rebuilt when its embedded source hash differs from
+    the sources (unless ``SKELFMM_AMD_NO_REBUILD`` is set: then an error)."""
+    global _TORCH_OPS
+    if _TORCH_OPS is not None:
+        return _TORCH_OPS
+    lib()  # the operator library links the HIP library
+    from . import _build
+
+    want = _build.source_hash()
+    have = _build.built_hash(TORCH_LIB_PATH)
+    if have != want:
+        if os.environ.get("SKELFMM_AMD_NO_REBUILD"):
+            raise RuntimeError(f"PyTorch operator library {TORCH_LIB_PATH} is missing or stale ({have}, sources "
+                               f"{want}); run __graft_entry__.build()")
+        _build.build_torch_ops()
+    torch.ops.load_library(TORCH_LIB_PATH)
+    _TORCH_OPS = torch.ops.skelfmm
+    return _TORCH_OPS
 
 
 def build_id():

@@ -88,3 +88,20 @@ def test_noalloc_near_field_rejects_workspace_arguments(hip_lib):
         a.urev_len = bad
         assert hip_lib.skf_p2p_noalloc(ctypes.byref(a), None) == 1
         assert b"outside" in hip_lib.skf_last_error()
+
+
+def test_torch_operator_registers_apply(hip_lib):
+    """The PyTorch operator library registers skelfmm::apply(int plan, Tensor x, Tensor(a!) y)
+    for GPU tensors only, with the HIP library's source hash; CPU tensors are refused by the
+    dispatcher before anything runs."""
+    import torch
+
+    from skelfmm_amd import _build
+
+    ops = hip.torch_ops()
+    assert _build.built_hash(hip.TORCH_LIB_PATH) == hip.build_id()
+    schema = str(ops.apply.default._schema)
+    assert schema.startswith("skelfmm::apply(int plan, Tensor x, Tensor(a!) y)")
+    x = torch.zeros(4, dtype=torch.float64)
+    with pytest.raises((NotImplementedError, RuntimeError)):
+        ops.apply(1, x, torch.zeros_like(x))

@@ -577,3 +577,31 @@ def test_coincident_points_in_both_forms(op_name):
     assert abs(y64[123] / 2.0 - k64) <= 1e-12 * abs(k64)
     assert abs(float(y32[123]) / 2.0 - k32) <= 1e-6 * abs(k32)
     assert y64[7] == 0.0 and float(y32[7]) == 0.0  # the charged point's own self pair is zero
+
+
+def test_torch_operator_matches_c_abi_and_checks_arguments():
+    """torch.ops.skelfmm.apply (the PyTorch extension) and skf_apply through ctypes on the same
+    plan give the same FP64 apply (atomic order aside), and the operator refuses mismatched
+    tensors with the reason."""
+    from skelfmm_amd import hip
+    from skelfmm_amd.batched import SkelFMMBatched
+
+    X = np.random.default_rng(9).random((3000, 2))
+    fmm = make_build(X, ops.LAPLACE_2D, tol=1e-6, leaf_size=64)
+    dev = SkelFMMBatched(fmm.tree, ops.LAPLACE_2D).build_from_copy(fmm)
+    x = torch.as_tensor(np.random.RandomState(1).randn(X.shape[0]), device=_gpu())
+    p = dev._plan(1)
+    y_op = torch.empty_like(x)
+    hip.torch_ops().apply(p["handle"].value, x, y_op)
+    y_c = torch.empty_like(x)
+    hip.check(hip.lib().skf_apply(p["handle"], hip.ptr(x), hip.ptr(y_c), hip.stream_handle(x.device)))
+    torch.cuda.synchronize()
+    assert rel_l2(y_op.cpu().numpy(), y_c.cpu().numpy()) <= 1e-14
+    ref = BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(x.cpu().numpy()).numpy()
+    assert rel_l2(y_op.cpu().numpy(), ref) <= FP64_TOL
+    with pytest.raises(RuntimeError, match="float32 or float64"):
+        hip.torch_ops().apply(p["handle"].value, x, torch.empty(x.shape, dtype=torch.float32, device=x.device))
+    with pytest.raises(RuntimeError, match="same size"):
+        hip.torch_ops().apply(p["handle"].value, x, torch.empty(10, dtype=x.dtype, device=x.device))
+    with pytest.raises(RuntimeError, match="null plan"):
+        hip.torch_ops().apply(0, x, y_op)
