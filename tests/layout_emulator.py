@@ -16,28 +16,36 @@ from oracle.kernels import GEOM_SIDE, kernel_matrix_torch
 
 
 def _near(lists, tpi, P, G, side, op_name, kappa, q, u, dt, symmetric):
+    """Every work item with all of its source entries at once (one kernel block per item)."""
+    e_pt = np.asarray(lists.e_pt, np.int64)
+    e_n = np.asarray(lists.e_n, np.int64)
+    e_q = np.asarray(lists.e_q, np.int64)
+    e_fl = np.asarray(lists.e_flags, np.int64)
     for g, t0, sb, se in lists.items:
         nt = min(tpi, int(lists.g_n[g]) - t0)
         tl = t0 + np.arange(nt)
         ts = lists.g_pt[g] + tl
-        acc = np.zeros(nt, dtype=dt)
-        for s in range(sb, se):
-            ss = lists.e_pt[s] + np.arange(lists.e_n[s])
-            XT = torch.as_tensor(P[:, ts].T)[None]
-            XS = torch.as_tensor(P[:, ss].T)[None]
-            gt = torch.as_tensor(G[:, ts].T)[None] if side == "target" else None
-            gs = torch.as_tensor(G[:, ss].T)[None] if side == "source" else None
-            K = kernel_matrix_torch(op_name, XT, XS, kappa, False, geom_trg=gt, geom_src=gs)[0].numpy()
-            if lists.e_flags[s] & 2:
-                assert not symmetric, "self segments stay one-sided"
-                lj = np.arange(lists.e_n[s])
-                K = np.where(tl[:, None] == lj[None, :], 0.0, K)
-            sgn = -1.0 if (lists.e_flags[s] & 1) else 1.0
-            jq = lists.e_q[s] + np.arange(lists.e_n[s])
-            acc += sgn * (K @ q[jq])
-            if symmetric:  # reverse direction: potentials of the entry at its charge slots
-                u[jq] += sgn * (K.T @ q[lists.g_out[g] + tl])
-        u[lists.g_out[g] + tl] += acc
+        ents = np.arange(sb, se)
+        cnt = e_n[ents]
+        first = np.repeat(np.cumsum(cnt) - cnt, cnt)
+        lj = np.arange(int(cnt.sum())) - first  # local index inside the entry
+        ent = np.repeat(ents, cnt)
+        ss = e_pt[ent] + lj
+        jq = e_q[ent] + lj
+        sgn = np.where(e_fl[ent] & 1, -1.0, 1.0)
+        XT = torch.as_tensor(P[:, ts].T)[None]
+        XS = torch.as_tensor(P[:, ss].T)[None]
+        gt = torch.as_tensor(G[:, ts].T)[None] if side == "target" else None
+        gs = torch.as_tensor(G[:, ss].T)[None] if side == "source" else None
+        K = kernel_matrix_torch(op_name, XT, XS, kappa, False, geom_trg=gt, geom_src=gs)[0].numpy()
+        selfcol = (e_fl[ent] & 2) != 0
+        if selfcol.any():
+            assert not symmetric, "self segments stay one-sided"
+            K = np.where(selfcol[None, :] & (tl[:, None] == lj[None, :]), 0.0, K)
+        K = K * sgn[None, :]
+        u[lists.g_out[g] + tl] += K @ q[jq]
+        if symmetric:  # reverse direction: potentials of the entries at their charge slots
+            np.add.at(u, jq, K.T @ q[lists.g_out[g] + tl])
 
 
 def emulate_apply(L, op_name, kappa, x, split=False):
