@@ -101,6 +101,7 @@ typedef struct skf_p2p_lists {
   const int* e_flags;
   int nitems;
   const int* items;
+  const float* e_shift; /* precision form, optional: as skf_p2p_args.e_shift, for these entries */
 } skf_p2p_lists;
 
 typedef struct skf_p2p_args {
@@ -154,6 +155,11 @@ typedef struct skf_p2p_args {
      pairs, far below the FP32 rounding of box-centred coordinates (~6e-8 of a box side) */
   const double* geo64;
   long long geo64_n;
+  /* precision form, optional: per entry the shift of its points into its group's box frame,
+     3 floats (centre of the group's box minus centre of the entry's box, formed in FP64 and
+     rounded once); NULL: the kernels form it from `centers` (one more dependent load when a work
+     item starts) */
+  const float* e_shift;
 } skf_p2p_args;
 
 /* one level of the upward or downward interpolation pass */

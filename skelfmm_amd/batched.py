@@ -190,6 +190,7 @@ class SkelFMMBatched(torch.nn.Module):
 
         self._nf = {k: i32(getattr(L, k)) for k in
                     ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags")}
+        self._nf["e_shift"] = self._entry_shifts(L)
         self.item_order = "heavy"
         self._nf["items"] = self._items_tensor(L)
         # symmetric split of the near field (single-layer kernels): pair lists + the rest
@@ -199,6 +200,7 @@ class SkelFMMBatched(torch.nn.Module):
             for part in L.split:
                 t = {k: i32(getattr(part, k)) for k in
                      ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags")}
+                t["e_shift"] = self._entry_shifts(part)
                 t["items"] = self._items_tensor(part)
                 self._split.append(t)
             self._split_structs = [hip.P2PLists(), hip.P2PLists()]
@@ -227,12 +229,22 @@ class SkelFMMBatched(torch.nn.Module):
         self._graphs = {}
         self.generation = getattr(self, "generation", 0) + 1
 
+    def _entry_shifts(self, lists):
+        """Precision form: per entry the shift of its points into its group's box frame (FP64
+        difference of the box centres rounded once to FP32, as the kernels would form it), or None."""
+        if self.precision != 1:
+            return None
+        C = np.asarray(self.layout.centers, dtype=np.float64)
+        gbox = np.repeat(np.asarray(lists.g_box, np.int64), np.asarray(lists.g_e1) - np.asarray(lists.g_e0))
+        sh = (C[gbox] - C[np.asarray(lists.e_box, np.int64)]).astype(np.float32)
+        return torch.as_tensor(np.ascontiguousarray(sh.reshape(-1)), device=self.device)
+
     def _fill_split_structs(self):
         for st, t in zip(self._split_structs, self._split):
             st.ngroups = int(t["g_pt"].shape[0])
             st.nentries = int(t["e_pt"].shape[0])
             st.nitems = int(t["items"].shape[0] // 4)
-            for k in ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags", "items"):
+            for k in ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags", "items", "e_shift"):
                 setattr(st, k, hip.ptr(t[k]))
 
     def set_sym_unroll(self, n):
@@ -265,7 +277,7 @@ class SkelFMMBatched(torch.nn.Module):
             a.geo64_n = int(self.geo64.shape[1])
         nf = self._nf
         a.ngroups = int(nf["g_pt"].shape[0])
-        for k in ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags", "items"):
+        for k in ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags", "items", "e_shift"):
             setattr(a, k, hip.ptr(nf[k]))
         a.nentries = int(nf["e_pt"].shape[0])
         a.nitems = int(nf["items"].shape[0] // 4)
@@ -540,10 +552,10 @@ class SkelFMMBatched(torch.nn.Module):
         if self.geo64 is not None:
             acc += self.geo64.numel() * 8
         for t in self._nf.values():
-            acc += t.numel() * t.element_size()
+            acc += 0 if t is None else t.numel() * t.element_size()
         for part in self._split or []:
             for t in part.values():
-                acc += t.numel() * t.element_size()
+                acc += 0 if t is None else t.numel() * t.element_size()
         for st in self._stages:
             for t in st.values():
                 acc += t.numel() * t.element_size()

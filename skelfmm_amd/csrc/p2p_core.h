@@ -239,7 +239,27 @@ struct P2PDev {
   // precision form of the double layers: FP64 rows [7][geo64_n] (x, y, z, nx, ny, nz, n.x)
   const double* geo64;
   long long geo64_n;
+  // precision form: per entry the shift of its points into its group's box frame (3 floats), or
+  // nullptr (then formed from the box centres in the kernel)
+  const float* e_shift;
 };
+
+// Shift of entry s's points into its group's box frame (precision form): the host's table when
+// there is one (one load level less at the start of a work item), else the FP64 difference of
+// the two box centres rounded once -- the same value the host stores.
+template <typename R>
+__device__ inline void entry_shift(const P2PDev& d, int s, double ctx, double cty, double ctz, R& ox, R& oy, R& oz) {
+  if (d.e_shift) {
+    ox = R(d.e_shift[3 * s + 0]);
+    oy = R(d.e_shift[3 * s + 1]);
+    oz = R(d.e_shift[3 * s + 2]);
+  } else {
+    const int eb = d.e_box[s];
+    ox = R(ctx - d.centers[3 * eb + 0]);
+    oy = R(cty - d.centers[3 * eb + 1]);
+    oz = R(ctz - d.centers[3 * eb + 2]);
+  }
+}
 
 // Normal projection rn = (x - y) . n of the double layers in the precision form. On a surface
 // rn ~ r^2 / 2 for close pairs while the FP32 box-centred coordinates carry ~6e-8 of a box side
@@ -384,7 +404,7 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
   double t64[4] = {0.0, 0.0, 0.0, 0.0};
   if constexpr (G64) Geo64<OP>::target(d.geo64, d.geo64_n, tslot, t64);
   double ctx = 0, cty = 0, ctz = 0;
-  if (PREC) {
+  if (PREC && !d.e_shift) {
     const int gb = d.g_box[g];
     ctx = d.centers[3 * gb + 0];
     cty = d.centers[3 * gb + 1];
@@ -403,10 +423,11 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
     // target coordinates relative to this source box (precision form)
     R xs0 = xt, ys0 = yt, zs0 = zt;
     if (PREC) {
-      const int eb = d.e_box[s];
-      xs0 = xt + R(ctx - d.centers[3 * eb + 0]);
-      ys0 = yt + R(cty - d.centers[3 * eb + 1]);
-      zs0 = zt + R(ctz - d.centers[3 * eb + 2]);
+      R ox, oy, oz;
+      entry_shift<R>(d, s, ctx, cty, ctz, ox, oy, oz);
+      xs0 = xt + ox;
+      ys0 = yt + oy;
+      zs0 = zt + oz;
     }
     for (int base = 0; base < ns; base += kWG) {
       // stage 256 sources (coordinates, normals, charges of this channel block)

@@ -249,10 +249,7 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
       // the centres) as they are staged, once per source for all targets of the item
       R ox = 0, oy = 0, oz = 0;
       if (PREC) {
-        const int eb = d.e_box[s];
-        ox = R(ctx - d.centers[3 * eb + 0]);
-        oy = R(cty - d.centers[3 * eb + 1]);
-        oz = R(ctz - d.centers[3 * eb + 2]);
+        entry_shift<R>(d, s, ctx, cty, ctz, ox, oy, oz);
       }
       for (int base = 0; base < ns; base += kWG) {
         const int nsc = min(kWG, ns - base);
@@ -320,7 +317,7 @@ __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
   if (se <= sb) return;  // padding item of an XCD-ordered table (uniform: the whole workgroup)
   const int ng = d.g_n[g];
   double ctx = 0, cty = 0, ctz = 0;
-  if (PREC) {
+  if (PREC && !d.e_shift) {
     const int gb = d.g_box[g];
     ctx = d.centers[3 * gb + 0];
     cty = d.centers[3 * gb + 1];
@@ -365,10 +362,7 @@ __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
       et.fl[e] = d.e_flags[s];
       R ox = 0, oy = 0, oz = 0;
       if (PREC) {
-        const int eb = d.e_box[s];
-        ox = R(ctx - d.centers[3 * eb + 0]);
-        oy = R(cty - d.centers[3 * eb + 1]);
-        oz = R(ctz - d.centers[3 * eb + 2]);
+        entry_shift<R>(d, s, ctx, cty, ctz, ox, oy, oz);
       }
       et.sh[e][0] = ox;
       et.sh[e][1] = oy;
@@ -481,7 +475,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
   double t64[4] = {0.0, 0.0, 0.0, 0.0};
   if constexpr (G64) Geo64<OP>::target(d.geo64, d.geo64_n, tslot, t64);
   double ctx = 0, cty = 0, ctz = 0;
-  if (PREC) {
+  if (PREC && !d.e_shift) {
     const int gb = d.g_box[g];
     ctx = d.centers[3 * gb + 0];
     cty = d.centers[3 * gb + 1];
@@ -532,10 +526,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
       e_meta[e][2] = d.e_flags[s];
       R ox = 0, oy = 0, oz = 0;
       if (PREC) {
-        const int eb = d.e_box[s];
-        ox = R(ctx - d.centers[3 * eb + 0]);
-        oy = R(cty - d.centers[3 * eb + 1]);
-        oz = R(ctz - d.centers[3 * eb + 2]);
+        entry_shift<R>(d, s, ctx, cty, ctz, ox, oy, oz);
       }
       e_shift[e][0] = ox;
       e_shift[e][1] = oy;
@@ -581,10 +572,7 @@ __global__ __launch_bounds__(kWG) void p2p_mfma_kernel(P2PDev d) {
         ept = d.e_pt[s];
         fl = d.e_flags[s];
         if (PREC) {
-          const int eb = d.e_box[s];
-          ox = R(ctx - d.centers[3 * eb + 0]);
-          oy = R(cty - d.centers[3 * eb + 1]);
-          oz = R(ctz - d.centers[3 * eb + 2]);
+          entry_shift<R>(d, s, ctx, cty, ctz, ox, oy, oz);
         }
       }
       const R sgn = (fl & SKF_SRC_NEG) ? R(-1) : R(1);
@@ -798,7 +786,7 @@ void p2p_mfma_dma_kernel(P2PDev d) {
   if constexpr (G64) Geo64<OP>::target(d.geo64, d.geo64_n, tslot, t64);
   // precision form: sources are shifted into the target box's frame (the FP64 form has no centres)
   double ctx = 0, cty = 0, ctz = 0;
-  if (PREC) {
+  if (PREC && !d.e_shift) {
     const int gb = d.g_box[g];
     ctx = d.centers[3 * gb + 0];
     cty = d.centers[3 * gb + 1];
@@ -855,10 +843,7 @@ void p2p_mfma_dma_kernel(P2PDev d) {
       e_meta[e][2] = d.e_flags[s];
       R ox = 0, oy = 0, oz = 0;
       if (PREC) {
-        const int eb = d.e_box[s];
-        ox = R(ctx - d.centers[3 * eb + 0]);
-        oy = R(cty - d.centers[3 * eb + 1]);
-        oz = R(ctz - d.centers[3 * eb + 2]);
+        entry_shift<R>(d, s, ctx, cty, ctz, ox, oy, oz);
       }
       e_shift[e][0] = ox;
       e_shift[e][1] = oy;
@@ -937,10 +922,7 @@ void p2p_mfma_dma_kernel(P2PDev d) {
         sl = d.e_pt[s] + j;
         fl = d.e_flags[s];
         if (PREC) {
-          const int eb = d.e_box[s];
-          ox = R(ctx - d.centers[3 * eb + 0]);
-          oy = R(cty - d.centers[3 * eb + 1]);
-          oz = R(ctz - d.centers[3 * eb + 2]);
+          entry_shift<R>(d, s, ctx, cty, ctz, ox, oy, oz);
         }
       }
       rself = (fl & SKF_SRC_SELF) ? j : -1;
@@ -1284,6 +1266,7 @@ void use_lists(P2PDev& d, const skf_p2p_lists* L) {
   d.g_pt = L->g_pt; d.g_n = L->g_n; d.g_out = L->g_out; d.g_box = L->g_box;
   d.e_pt = L->e_pt; d.e_n = L->e_n; d.e_q = L->e_q; d.e_box = L->e_box; d.e_flags = L->e_flags;
   d.items = reinterpret_cast<const int4*>(L->items);
+  d.e_shift = L->e_shift;  // precision form: the lists' own table (or none)
 }
 
 bool op_is_complex(int op) {
@@ -1334,6 +1317,7 @@ int run_p2p(const skf_p2p_args* a, const void* q, void* u, hipStream_t st) {
   d.urev = nullptr;
   d.geo64 = a->geo64;
   d.geo64_n = a->geo64_n;
+  d.e_shift = a->e_shift;
   // the 3D Helmholtz kernels take the branch-free reduced sincos (FP32: hardware sine/cosine after
   // the reduction; FP64: polynomials) when the host bounds every argument kappa r (kappa times
   // the largest pair distance) within its range

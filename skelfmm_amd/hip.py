@@ -65,7 +65,7 @@ class P2PLists(ctypes.Structure):
     _fields_ = [
         ("ngroups", _i), ("g_pt", _vp), ("g_n", _vp), ("g_out", _vp), ("g_box", _vp),
         ("nentries", _i), ("e_pt", _vp), ("e_n", _vp), ("e_q", _vp), ("e_box", _vp), ("e_flags", _vp),
-        ("nitems", _i), ("items", _vp),
+        ("nitems", _i), ("items", _vp), ("e_shift", _vp),
     ]
 
 
@@ -79,7 +79,7 @@ class P2PArgs(ctypes.Structure):
         ("nitems", _i), ("items", _vp), ("mfma_min_channels", _i),
         ("one", ctypes.POINTER(P2PLists)), ("sym", ctypes.POINTER(P2PLists)), ("urev", _vp),
         ("urev_len", ctypes.c_longlong), ("sym_unroll", _i), ("user_fn", _vp * 4),
-        ("max_distance", ctypes.c_double), ("geo64", _vp), ("geo64_n", ctypes.c_longlong),
+        ("max_distance", ctypes.c_double), ("geo64", _vp), ("geo64_n", ctypes.c_longlong), ("e_shift", _vp),
     ]
 
 
