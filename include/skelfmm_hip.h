@@ -160,6 +160,11 @@ typedef struct skf_p2p_args {
      rounded once); NULL: the kernels form it from `centers` (one more dependent load when a work
      item starts) */
   const float* e_shift;
+  /* 1: every output row of `items` belongs to exactly one work item (no group's sources are
+     split over several items) and u holds zero there before the call; the matrix-core kernel
+     (>= mfma_min_channels real channels, LDS-direct staging) then stores its sums instead of
+     adding them with atomics. 0: accumulate (the general contract) */
+  int store_rows;
 } skf_p2p_args;
 
 /* one level of the upward or downward interpolation pass */

@@ -192,6 +192,7 @@ class SkelFMMBatched(torch.nn.Module):
                     ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags")}
         self._nf["e_shift"] = self._entry_shifts(L)
         self.item_order = "heavy"
+        self.rows_exclusive = self._rows_exclusive(L)
         self._nf["items"] = self._items_tensor(L)
         # symmetric split of the near field (single-layer kernels): pair lists + the rest
         self._split = None
@@ -282,6 +283,8 @@ class SkelFMMBatched(torch.nn.Module):
         a.nentries = int(nf["e_pt"].shape[0])
         a.nitems = int(nf["items"].shape[0] // 4)
         a.mfma_min_channels = int(self.mfma_min_channels)
+        # the apply zeroes the potentials before the near field, so single-writer rows are stored
+        a.store_rows = 1 if self.rows_exclusive else 0
         a.sym_unroll = int(self.sym_unroll)
         a.max_distance = float(self.max_distance)
         uk = getattr(self.operator, "user_kernel", None)
@@ -466,7 +469,16 @@ class SkelFMMBatched(torch.nn.Module):
         items, _ = order_items(lists.items, lists.item_cost, self.item_order)
         return torch.as_tensor(np.ascontiguousarray(items.reshape(-1), dtype=np.int32), device=self.device)
 
+    @staticmethod
+    def _rows_exclusive(lists):
+        """Every (group, 64-target block) has one work item, i.e. no group's sources are split
+        over several items: each output row then has a single writer."""
+        it = np.asarray(lists.items, dtype=np.int64).reshape(-1, 4)
+        key = it[:, 0] * (1 << 31) + it[:, 1]
+        return bool(np.unique(key).size == key.size)
+
     def _upload_items(self):
+        self.rows_exclusive = self._rows_exclusive(self.layout)
         self._nf["items"] = self._items_tensor(self.layout)
         if self._split is not None:
             for t, part in zip(self._split, self.layout.split):

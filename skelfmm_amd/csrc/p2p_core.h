@@ -242,6 +242,8 @@ struct P2PDev {
   // precision form: per entry the shift of its points into its group's box frame (3 floats), or
   // nullptr (then formed from the box centres in the kernel)
   const float* e_shift;
+  // every output row belongs to one work item and holds zero: the matrix-core kernel stores
+  int store_rows;
 };
 
 // Shift of entry s's points into its group's box frame (precision form): the host's table when

@@ -1107,25 +1107,46 @@ void p2p_mfma_dma_kernel(P2PDev d) {
     tiles(IntC<2>{});
   else
     tiles(IntC<4>{});
-  if (!wave_busy) return;
-  fold();
   // The wave's 16 x 64 results go through LDS (every wave has passed the last barrier, so the
-  // tiles are no longer read; each wave takes a region of its own), so that each atomic
-  // instruction adds one target's 64 consecutive channels.
-  R* tr = sizeof(R) == 4 ? qa + wave * 16 * kMB : (wave < 2 ? qa : qb) + (wave & 1) * 16 * kMB;
+  // tiles are no longer read; each wave takes a region of its own), so that each store or atomic
+  // instruction covers one target's 64 consecutive channels.
+  auto region = [&](int w) SKF_INLINE {
+    return sizeof(R) == 4 ? qa + w * 16 * kMB : (w < 2 ? qa : qb) + (w & 1) * 16 * kMB;
+  };
+  auto to_lds = [&]() SKF_INLINE {
+    fold();
+    R* tr = region(wave);
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    R* trow = tr + MF::row(lane, r) * kMB + EPC * (lane & 15);
+    for (int r = 0; r < 4; ++r) {
+      R* trow = tr + MF::row(lane, r) * kMB + EPC * (lane & 15);
 #pragma unroll
-    for (int rd = 0; rd < 4 / EPC; ++rd) {
-      QV v;
+      for (int rd = 0; rd < 4 / EPC; ++rd) {
+        QV v;
 #pragma unroll
-      for (int e = 0; e < EPC; ++e) v[e] = FOLD ? R(acc[FOLD ? rd * EPC + e : 0][r]) : R(D[rd * EPC + e][r]);
-      *reinterpret_cast<QV*>(trow + rd * 16 * EPC) = v;
+        for (int e = 0; e < EPC; ++e) v[e] = FOLD ? R(acc[FOLD ? rd * EPC + e : 0][r]) : R(D[rd * EPC + e][r]);
+        *reinterpret_cast<QV*>(trow + rd * 16 * EPC) = v;
+      }
     }
-  }
+  };
   R* uo = static_cast<R*>(d.u);
   const int tb0 = t0 + tb * 16;
+  if (d.store_rows) {
+    // each output row belongs to this item alone and holds zero: the source parts of a target
+    // block are summed in LDS and the first part's wave stores the rows (no atomics)
+    if (wave_busy) to_lds();
+    __syncthreads();
+    if (!wave_busy || spart != 0) return;
+#pragma unroll 4
+    for (int t = 0; t < 16; ++t) {
+      R v = R(0);
+      for (int p = 0; p < SPL; ++p) v += region(p * TB + tb)[t * kMB + lane];
+      if (tb0 + t < ng && lane < ncv) uo[(long long)(d.g_out[g] + tb0 + t) * d.width + c0 + lane] = v;
+    }
+    return;
+  }
+  if (!wave_busy) return;
+  to_lds();
+  const R* tr = region(wave);
 #pragma unroll 4
   for (int t = 0; t < 16; ++t) {
     const R v = tr[t * kMB + lane];
@@ -1267,6 +1288,7 @@ void use_lists(P2PDev& d, const skf_p2p_lists* L) {
   d.e_pt = L->e_pt; d.e_n = L->e_n; d.e_q = L->e_q; d.e_box = L->e_box; d.e_flags = L->e_flags;
   d.items = reinterpret_cast<const int4*>(L->items);
   d.e_shift = L->e_shift;  // precision form: the lists' own table (or none)
+  d.store_rows = 0;         // the split lists' rows have several writers
 }
 
 bool op_is_complex(int op) {
@@ -1318,6 +1340,7 @@ int run_p2p(const skf_p2p_args* a, const void* q, void* u, hipStream_t st) {
   d.geo64 = a->geo64;
   d.geo64_n = a->geo64_n;
   d.e_shift = a->e_shift;
+  d.store_rows = a->store_rows;
   // the 3D Helmholtz kernels take the branch-free reduced sincos (FP32: hardware sine/cosine after
   // the reduction; FP64: polynomials) when the host bounds every argument kappa r (kappa times
   // the largest pair distance) within its range

@@ -80,6 +80,7 @@ class P2PArgs(ctypes.Structure):
         ("one", ctypes.POINTER(P2PLists)), ("sym", ctypes.POINTER(P2PLists)), ("urev", _vp),
         ("urev_len", ctypes.c_longlong), ("sym_unroll", _i), ("user_fn", _vp * 4),
         ("max_distance", ctypes.c_double), ("geo64", _vp), ("geo64_n", ctypes.c_longlong), ("e_shift", _vp),
+        ("store_rows", _i),
     ]
 
 

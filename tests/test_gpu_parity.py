@@ -605,3 +605,28 @@ def test_torch_operator_matches_c_abi_and_checks_arguments():
         hip.torch_ops().apply(p["handle"].value, x, torch.empty(10, dtype=x.dtype, device=x.device))
     with pytest.raises(RuntimeError, match="null plan"):
         hip.torch_ops().apply(0, x, y_op)
+
+
+@pytest.mark.parametrize("single", [False, True])
+def test_mfma_store_rows_and_atomic_sums_agree(single):
+    """Matrix-core near field with single-writer rows (one work item per target block: the sums
+    are stored, source parts combined in LDS) against split source runs (several items per block:
+    atomic sums): the same apply up to summation order; FP64 also against the oracle."""
+    from skelfmm_amd.batched import SkelFMMBatched
+
+    X = annulus_points(6000, 5)
+    fmm = make_build(X, ops.LAPLACE_2D, tol=1e-6, leaf_size=64)
+    dev = SkelFMMBatched(fmm.tree, ops.LAPLACE_2D).build_from_copy(fmm, use_single_precision=single)
+    Q = np.random.RandomState(11).randn(X.shape[0], 24)
+    x = torch.as_tensor(Q.astype(np.float32) if single else Q, device=_gpu())
+    dev.set_near_field_split(16384)
+    assert dev.rows_exclusive
+    y_store = dev.matvec(x).cpu().numpy()
+    dev.set_near_field_split(64)
+    assert not dev.rows_exclusive
+    y_atomic = dev.matvec(x).cpu().numpy()
+    assert rel_l2(y_store, y_atomic) <= (1e-6 if single else 1e-14)
+    if not single:
+        ref = BatchedApplyCPU(fmm, bytes_budget=2e8).matvec(Q).numpy()
+        assert rel_l2(y_store, ref) <= FP64_TOL
+        assert rel_l2(y_atomic, ref) <= FP64_TOL
