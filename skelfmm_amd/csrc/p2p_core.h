@@ -59,22 +59,22 @@ struct Mth<double> {
   __device__ static inline double by0(double x) { return y0(x); }
 };
 
-// sin and cos of an FP32 argument with |x| <= kTrigFastMax, without a branch: Cody-Waite
-// reduction x = n (2 pi) + y with 2 pi split in three FP32 parts (each product exact inside
-// the FMA), |y| <= pi, then the hardware sine and cosine of y / (2 pi) revolutions
-// (v_sin_f32 / v_cos_f32 on the transcendental unit; measured max absolute error 1.3e-7 over
-// [-1/2, 1/2] revolutions against FP64). Used by the FP32 form when the host bounds the kernel
-// argument (kappa times the largest pair distance); other arguments use sincosf.
+// FP32 sine and cosine of a bounded phase, without a branch. The host checks that every kernel
+// argument kappa r is at most kTrigFastMax and then selects these instances; their kernel
+// constant is kappa / (2 pi) (kernel_kappa), so the phase comes out in revolutions,
+// t = r kappa / (2 pi): one rounding of the constant and one of the product, as forming kappa r
+// has. t - rint(t) is exact (|t| < 2^22: the integer is 0, or both operands are FP32 within a
+// factor of 2 of each other), and the hardware v_sin_f32 / v_cos_f32 take revolutions in
+// [-1/2, 1/2] (measured max absolute error 1.3e-7 there against FP64 on an MI355X). Other
+// arguments use sincosf.
 constexpr float kTrigFastMax = 65536.0f;
+constexpr double kInvTwoPi = 0.15915494309189535;  // 1 / (2 pi)
+constexpr float kTwoPiF = 6.28318530717958648f;
 
-__device__ inline void sincos_reduced(float x, float* sp, float* cp) {
-  const float n = rintf(x * 0.159154936671257019f);
-  float y = fmaf(-n, 6.28318548202514648f, x);
-  y = fmaf(-n, -1.74845553146951720e-07f, y);
-  y = fmaf(-n, -6.86049804002352750e-15f, y);
-  const float t = y * 0.159154936671257019f;
-  *sp = __builtin_amdgcn_sinf(t);
-  *cp = __builtin_amdgcn_cosf(t);
+__device__ inline void sincos_rev(float t, float* sp, float* cp) {
+  const float y = t - rintf(t);
+  *sp = __builtin_amdgcn_sinf(y);
+  *cp = __builtin_amdgcn_cosf(y);
 }
 
 // The same for FP64 (the plain form), from the mathematics alone: Cody-Waite reduction
@@ -146,6 +146,19 @@ __device__ inline void sincos_k(R x, R* s, R* c) {
     Mth<R>::sc(x, s, c);
 }
 
+// FP32 instances with bounded arguments take the phase in revolutions (sincos_rev)
+template <typename R, bool FT>
+struct PhaseRev {
+  static constexpr bool on = FT && sizeof(R) == 4;
+};
+
+// The kernel constant a near-field kernel hands to kernel_value<R, OP, FT>: kappa, or kappa / (2 pi)
+// for the instances that take the phase in revolutions.
+template <typename R, bool FT>
+__device__ inline R kernel_kappa(double kappa) {
+  return PhaseRev<R, FT>::on ? R(kappa * kInvTwoPi) : R(kappa);
+}
+
 // instances with the reduced sincos exist for the 3D Helmholtz kernels (both precisions)
 template <typename R, int OP>
 struct HasFastTrig {
@@ -182,7 +195,10 @@ __device__ inline Cplx<R> kernel_value(R dx, R dy, R dz, R rn, R kappa) {
     R ir = M::rsq(rc);
     R r = rc * ir;
     R s, c;
-    sincos_k<R, FT>(kappa * r, &s, &c);
+    if constexpr (PhaseRev<R, FT>::on)
+      sincos_rev(r * kappa, &s, &c);  // kappa / (2 pi) here
+    else
+      sincos_k<R, FT>(kappa * r, &s, &c);
     k.re = c * ir;
     k.im = s * ir;
   } else if (OP == SKF_LAPLACE_DL_3D || OP == SKF_LAPLACE_ADL_3D) {
@@ -194,9 +210,15 @@ __device__ inline Cplx<R> kernel_value(R dx, R dy, R dz, R rn, R kappa) {
     R rc = M::fp32_bias ? r2 : fmax(r2, M::clamp_cd);
     R ir = M::rsq(rc);
     R r = rc * ir;
-    R s, c;
-    R kr = kappa * r;
-    sincos_k<R, FT>(kr, &s, &c);
+    R s, c, kr;
+    if constexpr (PhaseRev<R, FT>::on) {
+      const R t = r * kappa;  // kappa / (2 pi) here: revolutions
+      kr = t * kTwoPiF;
+      sincos_rev(t, &s, &c);
+    } else {
+      kr = kappa * r;
+      sincos_k<R, FT>(kr, &s, &c);
+    }
     R f = inv4pi * rn * ir * ir * ir;
     if (OP == SKF_HELMHOLTZ_ADL_3D) f = -f;
     // e^{ikr}(ikr - 1) = (c + i s)(-1 + i kr) = (-c - kr s) + i (kr c - s)
@@ -412,7 +434,7 @@ __global__ __launch_bounds__(kWG) void p2p_kernel(P2PDev d) {
     cty = d.centers[3 * gb + 1];
     ctz = d.centers[3 * gb + 2];
   }
-  const R kappa = R(d.kappa);
+  const R kappa = kernel_kappa<R, FT>(d.kappa);
 
   double acc[NA];
 #pragma unroll

@@ -128,7 +128,7 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
   const R* qv = static_cast<const R*>(d.q);
   R* uo = static_cast<R*>(d.u);
   const int w = d.width;
-  const R kappa = R(d.kappa);
+  const R kappa = kernel_kappa<R, FT>(d.kappa);
   // targets tp + lane + 64 k; padding targets repeat the first one with zero charge
   R qt[TPL * CW], xt[TPL], yt[TPL], zt[TPL];
 #pragma unroll

@@ -7,7 +7,9 @@ sources across entries up to 128 entries and stages entry by entry beyond. One g
 targets against NE single-point entries (every seventh subtracted) exercises both sides of
 those limits, for both forms and for widths that take each staging path (64 and 100:
 LDS-direct loads; 17 FP64 / 18 FP32: register staging), against a numpy FP64 sum of the same
-Laplace 2D kernel (0.5 log r^2).
+Laplace 2D kernel (0.5 log r^2). The 3D Laplace cases (1/r) run the symmetric kernel in both
+forms and at each inner-loop unroll on groups of up to 600 targets (several passes and work
+items, partial lanes) against entries of several points.
 """
 
 import ctypes
@@ -35,28 +37,35 @@ def _case(seed=0, ne=NE):
     return X, centers, box, sign
 
 
-def _run(precision, width, X, centers, box, sign, q, symmetric=False):
+def _run(precision, width, X, centers, box, sign, q, symmetric=False, kernel=0, nt=NT, e_n=None, sym_unroll=0):
     NE = sign.shape[0]
+    dim = X.shape[1]
+    e_n = np.ones(NE, np.int64) if e_n is None else np.asarray(e_n, np.int64)
+    e_pt = nt + np.concatenate(([0], np.cumsum(e_n)[:-1]))
     dev = torch.device("cuda")
     real = torch.float64 if precision == 0 else torch.float32
-    P = X - centers[box, :2] if precision == 1 else X  # the FP32 form stores box-centred points
+    P = X - centers[box, :dim] if precision == 1 else X  # the FP32 form stores box-centred points
     px = torch.as_tensor(P[:, 0], dtype=real, device=dev)
     py = torch.as_tensor(P[:, 1], dtype=real, device=dev)
+    pz = torch.as_tensor(P[:, 2], dtype=real, device=dev) if dim == 3 else None
     qd = torch.as_tensor(q, dtype=real, device=dev).contiguous()
     ud = torch.zeros_like(qd)
     cen = torch.as_tensor(centers, dtype=torch.float64, device=dev)
     i32 = lambda a: torch.as_tensor(np.asarray(a, np.int32), device=dev)
-    g = {"g_pt": i32([0]), "g_n": i32([NT]), "g_out": i32([0]), "g_box": i32([0])}
-    e = {"e_pt": i32(NT + np.arange(NE)), "e_n": i32(np.ones(NE)), "e_q": i32(NT + np.arange(NE)),
+    g = {"g_pt": i32([0]), "g_n": i32([nt]), "g_out": i32([0]), "g_box": i32([0])}
+    e = {"e_pt": i32(e_pt), "e_n": i32(e_n), "e_q": i32(e_pt),
          "e_box": i32(1 + np.arange(NE)), "e_flags": i32(np.where(sign < 0, 1, 0))}
-    items = i32([0, 0, 0, NE])
+    # one work item per 512 targets (the symmetric kernel's item size; the one-sided kernels take
+    # the group's first targets only in these tests)
+    nitems = (nt + 511) // 512 if symmetric else 1
+    items = i32([[0, 512 * i, 0, NE] for i in range(nitems)])
     a = hip.P2PArgs()
     keep = []
     if symmetric:
         # the whole list as pairs: targets get the sources' sums, every source the targets' sums
         one, sym = hip.P2PLists(), hip.P2PLists()
         one.ngroups = one.nentries = one.nitems = 0
-        sym.ngroups, sym.nentries, sym.nitems = 1, NE, 1
+        sym.ngroups, sym.nentries, sym.nitems = 1, NE, nitems
         for k, v in {**g, **e}.items():
             setattr(sym, k, hip.ptr(v))
         sym.items = hip.ptr(items)
@@ -64,8 +73,9 @@ def _run(precision, width, X, centers, box, sign, q, symmetric=False):
         keep = [one, sym, urev]
         a.one, a.sym = ctypes.pointer(one), ctypes.pointer(sym)
         a.urev, a.urev_len = (hip.ptr(urev), urev.numel()) if precision == 1 else (None, 0)
-    a.kernel, a.precision, a.width, a.kappa = 0, precision, width, 0.0
-    a.px, a.py, a.pz = hip.ptr(px), hip.ptr(py), None
+    a.kernel, a.precision, a.width, a.kappa = kernel, precision, width, 0.0
+    a.px, a.py, a.pz = hip.ptr(px), hip.ptr(py), (hip.ptr(pz) if pz is not None else None)
+    a.sym_unroll = sym_unroll
     a.centers = hip.ptr(cen) if precision == 1 else None
     a.q, a.u = hip.ptr(qd), hip.ptr(ud)
     a.ngroups, a.nentries, a.nitems = 1, NE, 1
@@ -110,3 +120,39 @@ def test_symmetric_near_field_stream_and_entry_paths(precision, ne):
     err = np.linalg.norm(u - ref) / np.linalg.norm(ref)
     print(f"symmetric precision {precision} entries {ne}: rel-l2 {err:.2e}", flush=True)
     assert err < (1e-13 if precision == 0 else 2e-6)
+
+
+def _case3(nt, ne, seed=0):
+    """One group of nt targets (box 0) against ne entries of 1..40 points, each in a box of its
+    own, every seventh entry subtracted; 3D points spread 0.02 around the box centres."""
+    rng = np.random.default_rng(seed)
+    centers = rng.random((ne + 1, 3))
+    e_n = rng.integers(1, 41, size=ne)
+    box = np.concatenate([np.zeros(nt, np.int64), np.repeat(1 + np.arange(ne), e_n)])
+    X = centers[box] + 0.02 * rng.normal(size=(box.shape[0], 3))
+    sign = np.where(np.arange(ne) % 7 == 3, -1.0, 1.0)
+    return X, centers, box, sign, e_n
+
+
+@pytest.mark.parametrize("precision,sym_unroll", [(0, 2), (1, 1), (1, 2), (1, 4)])
+@pytest.mark.parametrize("nt,ne", [(40, 100), (600, 30), (97, 300), (1, 5)])
+def test_symmetric_laplace3d_unrolls(precision, sym_unroll, nt, ne):
+    """3D Laplace symmetric pairs: every inner-loop unroll of the FP32 form and the FP64 form
+    against an FP64 sum, on one group of nt targets (one or two work items, passes of 256
+    targets with partial lanes) and ne entries of several points (up to 128 entries in the FP32
+    form: sources streamed across entries; beyond, and in the FP64 form: entry by entry)."""
+    X, centers, box, sign, e_n = _case3(nt, ne)
+    q = np.random.default_rng(3).normal(size=(X.shape[0], 1))
+    d = X[:nt, None, :] - X[None, nt:, :]
+    K = 1.0 / np.sqrt(np.sum(d * d, axis=-1)) * np.repeat(sign, e_n)[None, :]
+    ref = np.zeros_like(q)
+    ref[:nt] = K @ q[nt:]
+    ref[nt:] = K.T @ q[:nt]
+    print(f"3D symmetric precision {precision} unroll {sym_unroll}: {nt} targets, {ne} entries, "
+          f"{X.shape[0] - nt} sources", flush=True)
+    u = _run(precision, 1, X, centers, box, sign, q, symmetric=True, kernel=1, nt=nt, e_n=e_n, sym_unroll=sym_unroll)
+    err_t = np.linalg.norm(u[:nt] - ref[:nt]) / np.linalg.norm(ref[:nt])
+    err_s = np.linalg.norm(u[nt:] - ref[nt:]) / np.linalg.norm(ref[nt:])
+    print(f"  rel-l2 targets {err_t:.2e}, sources {err_s:.2e}", flush=True)
+    bar = 1e-13 if precision == 0 else 2e-6
+    assert err_t < bar and err_s < bar

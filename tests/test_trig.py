@@ -1,13 +1,17 @@
-"""The FP32 form's reduced-range sine/cosine (``sincos_reduced`` in ``csrc/p2p_core.h``).
+"""The device sine/cosine of the Helmholtz kernels (``csrc/p2p_core.h``).
 
-The constants are read from the header itself and the reduction (Cody-Waite by 2 pi in three
-FP32 parts, then y / (2 pi) revolutions for the hardware sine and cosine) is run in numpy FP32
-with each FMA rounded once: the revolutions handed to the hardware must equal the exact
-x / (2 pi) minus an integer to a few FP32 ulps of 1/2 over the whole range the host allows
-(|x| <= 65536), so a wrong digit, a dropped part of the split or a wrong scale fails here. The
-hardware sine/cosine itself (v_sin_f32 / v_cos_f32, measured max absolute error 1.3e-7 over
-[-1/2, 1/2] revolutions against FP64 on an MI355X) is checked end to end on the GPU
-(``test_gpu_parity.py::test_fp32_helmholtz_fast_trig_accuracy``).
+FP32 (``sincos_rev``): instances whose arguments the host bounds (kappa r <= kTrigFastMax) take
+kappa / (2 pi) as their constant, form the phase in revolutions t = r kappa / (2 pi), reduce it
+by t - rint(t) and hand it to the hardware sine and cosine (v_sin_f32 / v_cos_f32 take
+revolutions; measured max absolute error 1.3e-7 over [-1/2, 1/2] against FP64 on an MI355X,
+checked end to end on the GPU by ``test_gpu_parity.py::test_fp32_helmholtz_fast_trig_accuracy``).
+Here, in numpy FP32: the constants are 1/(2 pi) and 2 pi; the reduction is exact over the whole
+range the host allows; and the phase error of the whole path (constant and product each rounded
+once) stays within two roundings of the exact kappa r, as forming kappa r in FP32 does (a
+constant off by one FP32 ulp already fails the phase bound at the top of the range).
+
+FP64 (``sincos_reduced(double)``): Cody-Waite reduction by pi/2 and Taylor polynomials, run with
+exact FMAs against libm.
 """
 
 import os
@@ -18,73 +22,64 @@ import numpy as np
 
 HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "skelfmm_amd", "csrc",
                       "p2p_core.h")
+TRIG_FAST_MAX = 65536.0
 
 
-def _body():
+def _const(name):
     src = open(HEADER, encoding="utf-8").read()
-    a = src.index("__device__ inline void sincos_reduced(")
-    b = src.index("\n}\n", a)
-    return src[a:b]
+    m = re.search(r"constexpr (?:float|double) " + name + r" = (-?\d+\.\d+(?:e[-+]?\d+)?)f?;", src)
+    assert m, name
+    return float(m.group(1))
 
 
-def _consts():
-    return [float(m) for m in re.findall(r"(-?\d+\.\d+(?:e[-+]?\d+)?)f", _body())]
-
-
-def _fma(a, b, c):
-    return np.float32(np.float64(a) * np.float64(b) + np.float64(c))
-
-
-def _revolutions(x, k):
-    inv1, c1, c2, c3, inv2 = [np.float32(v) for v in k]
-    x = np.float32(x)
-    n = np.float32(np.rint(np.float32(x * inv1)))
-    y = _fma(-n, c1, x)
-    y = _fma(-n, c2, y)
-    y = _fma(-n, c3, y)
-    return np.float32(y * inv2), n
+def _phase_rev(r, kappa):
+    """The device path in FP32: constant kappa / (2 pi) rounded once, product rounded once,
+    reduction t - rint(t); returns (t, reduced revolutions)."""
+    krev = np.float32(np.float64(kappa) * _const("kInvTwoPi"))
+    t = np.float32(np.float32(r) * krev)
+    return t, np.float32(t - np.float32(np.rint(t)))
 
 
 def test_constants_are_the_mathematical_ones():
-    k = _consts()
-    assert len(k) == 5
     getcontext().prec = 50
     pi = Decimal("3.14159265358979323846264338327950288419716939937510")
-    two_pi = 2 * pi
-    for inv in (k[0], k[4]):
-        assert abs(Decimal(float(np.float32(inv))) - 1 / two_pi) < Decimal("1e-8")
-    split = sum(Decimal(float(np.float32(c))) for c in k[1:4])
-    assert abs(split - two_pi) < Decimal("1e-21"), "2 pi must be split exactly enough for |n| up to 1.1e4"
-    assert float(np.float32(k[1])) == float(np.float32(2 * np.pi))
+    assert abs(Decimal(_const("kInvTwoPi")) - 1 / (2 * pi)) < Decimal("1e-17")
+    assert _const("kTwoPiF") == float(2 * pi) and np.float32(_const("kTwoPiF")) == np.float32(2 * np.pi)
+    assert _const("kTrigFastMax") == TRIG_FAST_MAX
 
 
-def test_reduction_matches_exact_revolutions_over_the_allowed_range():
-    k = _consts()
+def test_reduction_is_exact_over_the_allowed_range():
     rng = np.random.default_rng(0)
-    xs = np.concatenate([rng.uniform(-8.0, 8.0, 3000), rng.uniform(-65536.0, 65536.0, 3000),
-                         np.arange(-40, 41) * (np.pi / 4), [0.0, 1e-30, 65536.0, -65536.0]]).astype(np.float32)
-    getcontext().prec = 40
-    two_pi = 2 * Decimal("3.14159265358979323846264338327950288419716939937510")
-    worst = 0.0
-    for x in xs:
-        t, n = _revolutions(x, k)
-        exact = Decimal(float(x)) / two_pi - Decimal(float(n))  # the FP32 input is exact
-        worst = max(worst, abs(float(Decimal(float(t)) - exact)))
-        # n = rint(fl(x / 2 pi)) may pick the neighbour near a half turn: |t| <= 1/2 + |x| 2^-24 / (2 pi)
-        assert abs(float(t)) <= 0.5 + 1e-3
-        # with an ideal sine of the revolutions the value is within the reduction error
-        assert abs(np.sin(2 * np.pi * float(t)) - np.sin(float(x))) < 4e-7
-    assert worst < 6e-8, worst
+    tmax = TRIG_FAST_MAX / (2 * np.pi)
+    ts = np.concatenate([rng.uniform(0.0, 2.0, 4000), rng.uniform(0.0, tmax, 4000),
+                         np.arange(0, 41) * 0.25, [0.0, 1e-30, 0.5, 1.5, 2.5, tmax]]).astype(np.float32)
+    y = ts - np.rint(ts)  # FP32 arithmetic
+    exact = ts.astype(np.float64) - np.rint(ts.astype(np.float64))
+    assert np.array_equal(y.astype(np.float64), exact)
+    assert np.all(np.abs(y) <= 0.5)
 
 
-def test_quadrants_and_signs():
-    k = _consts()
-    for q in range(-8, 9):
-        x = np.float32(q * np.pi / 2 + 0.3)
-        t, _ = _revolutions(x, k)
-        xd = float(x)
-        assert abs(np.sin(2 * np.pi * float(t)) - np.sin(xd)) < 3e-7
-        assert abs(np.cos(2 * np.pi * float(t)) - np.cos(xd)) < 3e-7
+def test_phase_error_is_two_roundings_of_kappa_r():
+    """Against the exact phase kappa r (FP64 for the FP32 inputs r, kappa): at most two FP32
+    roundings of kappa r plus the constant's, over the range the host allows, and never worse
+    than forming kappa r in FP32 and reducing it exactly."""
+    rng = np.random.default_rng(1)
+    worst_new = worst_ref = 0.0
+    for kappa in (1.0, 10 * np.pi, 20.0, 40.0, 1000.0):
+        rmax = TRIG_FAST_MAX / kappa
+        r = np.concatenate([rng.uniform(0.0, min(rmax, 1.0), 2000), rng.uniform(0.0, rmax, 2000)]).astype(np.float32)
+        t, y = _phase_rev(r, kappa)
+        exact = (r.astype(np.float64) * np.float64(np.float32(kappa))) / (2 * np.pi)  # revolutions
+        err = np.abs(((y.astype(np.float64) - exact) + 0.5) % 1.0 - 0.5) * 2 * np.pi  # radians
+        bound = 3 * 2.0**-24 * exact * 2 * np.pi + 1e-12
+        assert np.all(err <= bound), float(np.max(err / bound))
+        kr = (r * np.float32(kappa)).astype(np.float32)  # kappa r formed in FP32, reduced exactly
+        err_ref = np.abs(((kr.astype(np.float64) / (2 * np.pi) - exact) + 0.5) % 1.0 - 0.5) * 2 * np.pi
+        worst_new = max(worst_new, float(np.max(err / np.maximum(exact, 1.0))))
+        worst_ref = max(worst_ref, float(np.max(err_ref / np.maximum(exact, 1.0))))
+        # with an ideal sine of the reduced revolutions, sin and cos follow the phase
+        assert np.max(np.abs(np.sin(2 * np.pi * y.astype(np.float64)) - np.sin(2 * np.pi * exact))) <= np.max(bound) + 1e-9
+    assert worst_new <= 2.0 * worst_ref + 1e-12, (worst_new, worst_ref)
 
 
 # ------------------------------------------------------------------ FP64 (the plain form)
