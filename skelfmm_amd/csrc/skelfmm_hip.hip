@@ -60,10 +60,13 @@ extern "C" const char* skf_build_id(void) { return SKF_SOURCE_HASH; }
 
 namespace {
 
-// ------------------------------------------- symmetric near field (single-layer kernels)
-// For two point sets U, V that interact in both directions with the same sign and a symmetric
-// kernel (K(x, y) = K(y, x)), one evaluation of K(t, s) serves u_U[t] += K q_V[s] and
-// u_V[s] += K q_U[t]. A work item holds up to kSymTargets targets of one group, taken in passes
+// ------------------------------------------- symmetric near field (pairs of lists)
+// For two point sets U, V that interact in both directions with the same sign, one evaluation per
+// point pair serves u_U[t] += K(t, s) q_V[s] and u_V[s] += K(s, t) q_U[t]. Single layers are
+// symmetric, K(s, t) = K(t, s). Double layers are K(x, y) = g(|x - y|) rn with the normal
+// projection rn linear in the normal (source normal: (x - y) . n_y; target normal: (x - y) . n_x),
+// so g is evaluated once (kernel_value at rn = 1) and scaled by the two directions' projections.
+// A work item holds up to kSymTargets targets of one group, taken in passes
 // of 64 * TPL (TPL per lane, in registers, with their own charges q[g_out + t]), and a run of
 // its source entries; the
 // 4 waves split every 256-source tile. Forward sums stay in the lane (FP64 across tiles, combined
@@ -74,11 +77,29 @@ namespace {
 constexpr int kSymTargets = 512;  // targets per work item (the host's SYM_TARGETS_PER_ITEM)
 
 // targets per lane and pass: 256-target passes fit the typical leaf (a few hundred points)
-// without idle lanes and keep the registers at four waves per SIMD or more
-template <typename R>
+// without idle lanes and keep the registers at four waves per SIMD or more; double layers hold
+// each target's geometry too and take two per lane
+template <typename R, int OP>
 struct SymTPL {
-  static constexpr int n = 4;
+  static constexpr int n = OpT<OP>::geom ? 2 : 4;
 };
+
+// Staged source row of the symmetric kernel: coordinates, charges and, for the plain form of a
+// double layer, the normal (both directions' projections need both points' normals). The
+// precision form of a double layer keeps each point's FP64 row (x, y, z, nx, ny, nz, n . x) in
+// registers (targets) or in an LDS tile of its own (sources) instead.
+template <int OP, bool PREC>
+struct SymRow {
+  static constexpr int dim = OpT<OP>::dim;
+  static constexpr int cw = OpT<OP>::cplx ? 2 : 1;
+  static constexpr bool g64 = PREC && OpT<OP>::geom != 0;
+  static constexpr int nrm = (OpT<OP>::geom != 0 && !PREC) ? 3 : 0;  // normal at dim + cw
+  static constexpr int nv = (dim + cw + nrm + 1) & ~1;
+  // offsets of a point's "target values" and "source values" in its FP64 row (Geo64<OP>::rn)
+  static constexpr int toff = OpT<OP>::geom == 1 ? 0 : 3;
+  static constexpr int soff = OpT<OP>::geom == 1 ? 3 : 0;
+};
+constexpr int kGeoRow = 7;  // FP64 geometry row: x, y, z, nx, ny, nz, n . x
 
 // sources per reverse block: the per-wave table (n x 64 partials) stays at 4 KiB
 template <typename R, int CW>
@@ -109,13 +130,16 @@ struct SymEnt {
 
 // One pass of the symmetric kernel over targets tp .. tp + 64 TPL - 1 of the item (TPL per lane).
 template <typename R, int OP, bool PREC, int UNR, int TPL, bool FT>
-__device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, R* __restrict__ rvl,
-                                         double* __restrict__ red, int* __restrict__ sidx, const SymEnt<R>* et,
-                                         bool etab, int nent, int tot, int tp, int tend, int t0, int sb, int se,
-                                         double ctx, double cty, double ctz) {
-  using S = Stage<OP, 1>;
+__device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, double* __restrict__ gtile,
+                                         R* __restrict__ rvl, double* __restrict__ red, int* __restrict__ sidx,
+                                         const SymEnt<R>* et, bool etab, int nent, int tot, int tp, int tend, int t0,
+                                         int sb, int se, double ctx, double cty, double ctz) {
+  using S = SymRow<OP, PREC>;
   constexpr int NV = S::nv;
   constexpr int CW = S::cw;
+  constexpr int GEOM = OpT<OP>::geom;
+  constexpr bool G64 = S::g64;
+  constexpr int NRM = S::nrm;
   constexpr int SUB = SymSub<R, CW>::n;
   constexpr int LPS = SymSub<R, CW>::lanes;
   const int tid = threadIdx.x, lane = tid & 63;
@@ -131,6 +155,8 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
   const R kappa = kernel_kappa<R, FT>(d.kappa);
   // targets tp + lane + 64 k; padding targets repeat the first one with zero charge
   R qt[TPL * CW], xt[TPL], yt[TPL], zt[TPL];
+  R nt[NRM ? TPL : 1][3];            // plain form of a double layer: target normals
+  double gt[G64 ? TPL : 1][kGeoRow];  // precision form of a double layer: target FP64 rows
 #pragma unroll
   for (int k = 0; k < TPL; ++k) {
     const int t = tp + lane + 64 * k;
@@ -140,6 +166,15 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
     zt[k] = (S::dim == 3) ? pz[sl] : R(0);
 #pragma unroll
     for (int c = 0; c < CW; ++c) qt[k * CW + c] = (t < tend) ? qv[(long long)(gout + t) * w + c] : R(0);
+    if constexpr (NRM != 0) {
+      nt[k][0] = static_cast<const R*>(d.nx)[sl];
+      nt[k][1] = static_cast<const R*>(d.ny)[sl];
+      nt[k][2] = static_cast<const R*>(d.nz)[sl];
+    }
+    if constexpr (G64) {
+#pragma unroll
+      for (int c = 0; c < kGeoRow; ++c) gt[k][c] = d.geo64[c * d.geo64_n + sl];
+    }
   }
   double acc[TPL * CW];
 #pragma unroll
@@ -152,6 +187,7 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
     const int wb = wave * 64;  // wave-uniform, within the tile
     const int cnt = __builtin_amdgcn_readfirstlane(max(0, min(64, nsc - wb)));
     const R* tw = &tile[wave * 64 * NV];
+    const double* gw = G64 ? &gtile[wave * 64 * kGeoRow] : nullptr;
 #pragma unroll
     for (int a = 0; a < TPL * CW; ++a) part[a] = 0;
     for (int sub = 0; sub < cnt; sub += SUB) {
@@ -169,18 +205,39 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
           const R dx = xt[k] - xs;
           const R dy = yt[k] - ys;
           const R dz = (S::dim == 3) ? zt[k] - zs : R(0);
-          const Cplx<R> kv = kernel_value<R, OP, FT>(dx, dy, dz, R(0), kappa);
+          // single layer: one value both ways; double layer: g(r) (the value at rn = 1) scaled by
+          // the forward projection (point t against source s) and the reverse one (s against t)
+          const Cplx<R> kv = kernel_value<R, OP, FT>(dx, dy, dz, GEOM ? R(1) : R(0), kappa);
+          Cplx<R> kf = kv, kr = kv;
+          if constexpr (GEOM != 0) {
+            R rf, rr;
+            if constexpr (G64) {
+              const double* gs = gw + (sub + jj) * kGeoRow;
+              rf = Geo64<OP>::rn(gt[k] + S::toff, gs + S::soff);
+              rr = Geo64<OP>::rn(gs + S::toff, gt[k] + S::soff);
+            } else {
+              const R* ns = row + S::dim + CW;
+              const R pns = dx * ns[0] + dy * ns[1] + dz * ns[2];                // (t - s) . n_s
+              const R pnt = dx * nt[k][0] + dy * nt[k][1] + dz * nt[k][2];       // (t - s) . n_t
+              rf = GEOM == 1 ? pns : pnt;
+              rr = GEOM == 1 ? -pnt : -pns;
+            }
+            kf.re = kv.re * rf;
+            kf.im = kv.im * rf;
+            kr.re = kv.re * rr;
+            kr.im = kv.im * rr;
+          }
           if (CW == 1) {
-            part[k] += kv.re * row[S::dim];
-            r0 += kv.re * qt[k];
+            part[k] += kf.re * row[S::dim];
+            r0 += kr.re * qt[k];
           } else {
             // complex multiply-accumulates as two FMAs each (no separate product and add)
             const R sr = row[S::dim], si = row[S::dim + 1];
-            part[2 * k] = fma(kv.re, sr, fma(-kv.im, si, part[2 * k]));
-            part[2 * k + 1] = fma(kv.re, si, fma(kv.im, sr, part[2 * k + 1]));
+            part[2 * k] = fma(kf.re, sr, fma(-kf.im, si, part[2 * k]));
+            part[2 * k + 1] = fma(kf.re, si, fma(kf.im, sr, part[2 * k + 1]));
             const R tr = qt[2 * k], ti = qt[2 * k + 1];
-            r0 = fma(kv.re, tr, fma(-kv.im, ti, r0));
-            r1 = fma(kv.re, ti, fma(kv.im, tr, r1));
+            r0 = fma(kr.re, tr, fma(-kr.im, ti, r0));
+            r1 = fma(kr.re, ti, fma(kr.im, tr, r1));
           }
         }
         rvl[((wave * SUB + jj) * CW + 0) * 64 + lane] = r0;
@@ -200,6 +257,25 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
         if (pp == 0 && sub + j < cnt) dst(wb + sub + j, c, double(v));
       }
       wave_sync_lds();
+    }
+  };
+  // a staged source's geometry (double layers): its normal in the row (plain form) or its FP64 row
+  auto stage_geom = [&](R* row, int i, int sl) {
+    if constexpr (NRM != 0) {
+      row[S::dim + CW + 0] = static_cast<const R*>(d.nx)[sl];
+      row[S::dim + CW + 1] = static_cast<const R*>(d.ny)[sl];
+      row[S::dim + CW + 2] = static_cast<const R*>(d.nz)[sl];
+    }
+    if constexpr (G64) {
+#pragma unroll
+      for (int c = 0; c < kGeoRow; ++c) gtile[i * kGeoRow + c] = d.geo64[c * d.geo64_n + sl];
+    }
+  };
+  // padding rows: zero charges; a finite geometry (its terms are multiplied by zero)
+  auto stage_geom_pad = [&](int i) {
+    if constexpr (G64) {
+#pragma unroll
+      for (int c = 0; c < kGeoRow; ++c) gtile[i * kGeoRow + c] = 0.0;
     }
   };
   if constexpr (sizeof(R) == 8 && !OpT<OP>::cplx) etab = false;  // entry by entry (see the kernel)
@@ -225,10 +301,12 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
           const R* qs = qv + (long long)qs0 * w;
 #pragma unroll
           for (int c = 0; c < CW; ++c) row[S::dim + c] = neg ? -qs[c] : qs[c];
+          stage_geom(row, tid, sl);
           sidx[tid] = neg ? ~qs0 : qs0;
         } else {
 #pragma unroll
           for (int c = 0; c < NV; ++c) row[c] = R(0);
+          stage_geom_pad(tid);
         }
       }
       __syncthreads();
@@ -264,9 +342,11 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
             const R* qs = qv + (long long)(qo + j) * w;
 #pragma unroll
             for (int c = 0; c < CW; ++c) row[S::dim + c] = qs[c];
+            stage_geom(row, tid, sl);
           } else {
 #pragma unroll
             for (int c = 0; c < NV; ++c) row[c] = R(0);
+            stage_geom_pad(tid);
           }
         }
         __syncthreads();
@@ -300,13 +380,13 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
 
 template <typename R, int OP, bool PREC, int UNR, bool FT = false>
 __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
-  using S = Stage<OP, 1>;
+  using S = SymRow<OP, PREC>;
   constexpr int NV = S::nv;
   constexpr int CW = S::cw;
-  constexpr int TPL = SymTPL<R>::n;
+  constexpr int TPL = SymTPL<R, OP>::n;
   constexpr int SUB = SymSub<R, CW>::n;
-  static_assert(OpT<OP>::geom == 0, "symmetric evaluation needs a single-layer kernel");
   __shared__ R tile[kWG * NV];
+  __shared__ double gtile[S::g64 ? kWG * kGeoRow : 1];  // precision form of a double layer
   __shared__ R rvl[4 * SUB * CW * 64];  // reverse partials of one block of sources, per wave
   __shared__ double red[4 * kTCH * CW];
   __shared__ SymEnt<R> et;
@@ -376,14 +456,18 @@ __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
   // beyond the next multiple of 64)
   for (int tp = t0; tp < tend;) {
     const int tpl = min(TPL, (tend - tp + 63) / 64);
-    if (tpl >= 4)
-      sym_pass<R, OP, PREC, UNR, 4, FT>(d, tile, rvl, red, sidx, &et, etab, nent, tot, tp, tend, t0, sb, se, ctx, cty, ctz);
-    else if (tpl == 3)
-      sym_pass<R, OP, PREC, UNR, 3, FT>(d, tile, rvl, red, sidx, &et, etab, nent, tot, tp, tend, t0, sb, se, ctx, cty, ctz);
-    else if (tpl == 2)
-      sym_pass<R, OP, PREC, UNR, 2, FT>(d, tile, rvl, red, sidx, &et, etab, nent, tot, tp, tend, t0, sb, se, ctx, cty, ctz);
+    if (TPL >= 4 && tpl >= 4)
+      sym_pass<R, OP, PREC, UNR, (TPL >= 4 ? 4 : 1), FT>(d, tile, gtile, rvl, red, sidx, &et, etab, nent, tot, tp, tend,
+                                                          t0, sb, se, ctx, cty, ctz);
+    else if (TPL >= 3 && tpl == 3)
+      sym_pass<R, OP, PREC, UNR, (TPL >= 3 ? 3 : 1), FT>(d, tile, gtile, rvl, red, sidx, &et, etab, nent, tot, tp, tend,
+                                                          t0, sb, se, ctx, cty, ctz);
+    else if (tpl >= 2)
+      sym_pass<R, OP, PREC, UNR, 2, FT>(d, tile, gtile, rvl, red, sidx, &et, etab, nent, tot, tp, tend, t0, sb, se, ctx,
+                                        cty, ctz);
     else
-      sym_pass<R, OP, PREC, UNR, 1, FT>(d, tile, rvl, red, sidx, &et, etab, nent, tot, tp, tend, t0, sb, se, ctx, cty, ctz);
+      sym_pass<R, OP, PREC, UNR, 1, FT>(d, tile, gtile, rvl, red, sidx, &et, etab, nent, tot, tp, tend, t0, sb, se, ctx,
+                                        cty, ctz);
     tp += 64 * tpl;
   }
 }
@@ -1255,6 +1339,24 @@ int launch_p2p_sym_u(int op, const P2PDev& d, int nitems, hipStream_t st) {
       else
         hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_HELMHOLTZ_3D, PREC, UNR>), grid, block, 0, st, d);
       break;
+    case SKF_LAPLACE_DL_3D:
+      hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_LAPLACE_DL_3D, PREC, UNR>), grid, block, 0, st, d);
+      break;
+    case SKF_LAPLACE_ADL_3D:
+      hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_LAPLACE_ADL_3D, PREC, UNR>), grid, block, 0, st, d);
+      break;
+    case SKF_HELMHOLTZ_DL_3D:
+      if (d.fast_trig)
+        hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_HELMHOLTZ_DL_3D, PREC, UNR, true>), grid, block, 0, st, d);
+      else
+        hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_HELMHOLTZ_DL_3D, PREC, UNR>), grid, block, 0, st, d);
+      break;
+    case SKF_HELMHOLTZ_ADL_3D:
+      if (d.fast_trig)
+        hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_HELMHOLTZ_ADL_3D, PREC, UNR, true>), grid, block, 0, st, d);
+      else
+        hipLaunchKernelGGL((p2p_sym_kernel<R, SKF_HELMHOLTZ_ADL_3D, PREC, UNR>), grid, block, 0, st, d);
+      break;
     default: return set_err(SKF_ERR_ARG, "kernel %d has no symmetric form", op);
   }
   SKF_LAUNCHED();
@@ -1271,8 +1373,10 @@ int launch_p2p_sym(int op, const P2PDev& d, int nitems, hipStream_t st, int unro
   return set_err(SKF_ERR_ARG, "symmetric unroll must be 0, 1, 2 or 4");
 }
 
+// kernels whose pairs of lists share one evaluation per point pair (p2p_sym_kernel)
 bool op_symmetric(int op) {
-  return op == SKF_LAPLACE_2D || op == SKF_LAPLACE_3D || op == SKF_HELMHOLTZ_2D || op == SKF_HELMHOLTZ_3D;
+  return op == SKF_LAPLACE_2D || op == SKF_LAPLACE_3D || op == SKF_HELMHOLTZ_2D || op == SKF_HELMHOLTZ_3D ||
+         op == SKF_LAPLACE_DL_3D || op == SKF_LAPLACE_ADL_3D || op == SKF_HELMHOLTZ_DL_3D || op == SKF_HELMHOLTZ_ADL_3D;
 }
 
 int check_lists(const skf_p2p_lists* L) {

@@ -8,7 +8,7 @@ Every array equals the host statement's bit for bit (``tests/test_gpu_tree.py``)
 
 * rows (target box, target kind, source box, source kind, charge kind, subtract) of every
   list, rows with an empty target or source set dropped;
-* the symmetric split (single-layer kernels): a row pairs with its mirror when both are unique
+* the symmetric split (kernels of ``layout.SYMMETRIC_KERNEL_IDS``): a row pairs with its mirror when both are unique
   and each charge kind equals its source set's kind; the representative is the row whose target
   set orders first;
 * per table: rows stably sorted by (target box, target kind, source box), one group per target

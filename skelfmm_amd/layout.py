@@ -473,9 +473,11 @@ class NearLists:
               "e_flags")
 
 
-# kernels with K(x, y) = K(y, x) (single layers): pairs of lists that run in both directions are
-# evaluated once by the symmetric near-field kernel
-SYMMETRIC_KERNEL_IDS = (0, 1, 2, 3)
+# kernels whose pairs of lists (rows U -> V and V -> U) are evaluated once per point pair by the
+# symmetric near-field kernel: single layers, K(x, y) = K(y, x), and the double layers,
+# K(x, y) = g(|x - y|) (x - y) . n with n the normal at y (double layer) or x (adjoint), whose
+# two directions share g and differ in the projection
+SYMMETRIC_KERNEL_IDS = (0, 1, 2, 3, 4, 5, 6, 7)
 SYM_TARGETS_PER_ITEM = 512
 
 

@@ -3,7 +3,7 @@
 Executes exactly the index maps and work lists the HIP kernels consume
 (charge gather, upward gather + GEMV per stage, near-field work items with the
 SRC_NEG / SRC_SELF flags -- or the symmetric split: one-sided rest plus pair entries
-evaluated in both directions -- downward parent step + GEMV^T, result scatter), with
+evaluated in both directions, K(s, t) for the reverse one -- downward parent step + GEMV^T, result scatter), with
 dense blocks from the oracle's clamped torch kernels. Agreement with the
 oracle's padded apply validates the layout on the CPU; agreement of the GPU
 with the oracle then isolates the kernels.
@@ -45,7 +45,14 @@ def _near(lists, tpi, P, G, side, op_name, kappa, q, u, dt, symmetric):
         K = K * sgn[None, :]
         u[lists.g_out[g] + tl] += K @ q[jq]
         if symmetric:  # reverse direction: potentials of the entries at their charge slots
-            np.add.at(u, jq, K.T @ q[lists.g_out[g] + tl])
+            if side is None:
+                Kr = K.T  # single layers: K(s, t) = K(t, s)
+            else:  # double layers: K(s, t), the two sets' roles and normals exchanged
+                gt2 = torch.as_tensor(G[:, ss].T)[None] if side == "target" else None
+                gs2 = torch.as_tensor(G[:, ts].T)[None] if side == "source" else None
+                Kr = kernel_matrix_torch(op_name, XS, XT, kappa, False, geom_trg=gt2, geom_src=gs2)[0].numpy()
+                Kr = Kr * sgn[:, None]
+            np.add.at(u, jq, Kr @ q[lists.g_out[g] + tl])
 
 
 def emulate_apply(L, op_name, kappa, x, split=False):

@@ -481,11 +481,14 @@ def test_item_orders_keep_parity(name, single, nrhs):
         dev.set_item_order("random")
 
 
-@pytest.mark.parametrize("name", ["laplace_2d_annulus", "laplace_3d_cube", "helmholtz_3d_cube", "helmholtz_2d_square"])
+@pytest.mark.parametrize("name", ["laplace_2d_annulus", "laplace_3d_cube", "helmholtz_3d_cube", "helmholtz_2d_square",
+                                  "laplace_dl_sphere", "laplace_adl_sphere", "helmholtz_dl_sphere",
+                                  "helmholtz_adl_sphere"])
 @pytest.mark.parametrize("single", [False, True])
 def test_symmetric_near_field_matches_one_sided(name, single):
     """Pairs evaluated once for both directions (p2p_sym_kernel) against the one-sided kernel:
-    FP64 to 1e-13 (only the summation order differs), FP32 form to 3e-6 of the result."""
+    FP64 to 1e-13 (only the summation order differs), FP32 form to 3e-6 of the result. Double
+    layers share g(r) between the directions and take each direction's normal projection."""
     from skelfmm_amd.batched import SkelFMMBatched
 
     X, op, kappa, geom, leaf = _case(name)

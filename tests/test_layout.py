@@ -7,7 +7,7 @@ from conftest import annulus_points, make_build, rel_l2, sphere_points
 from layout_emulator import emulate_apply
 from oracle.batched_apply import BatchedApplyCPU
 from skelfmm_amd import operators as ops
-from skelfmm_amd.layout import N_XCD, SRC_NEG, SRC_SELF, build_layout, order_items
+from skelfmm_amd.layout import N_XCD, SRC_NEG, SRC_SELF, SYMMETRIC_KERNEL_IDS, build_layout, order_items
 
 
 def _builds():
@@ -145,8 +145,10 @@ def test_merged_layout_emulates_every_instance():
     assert levels == sorted(set(levels), reverse=True)
 
 
-SYMMETRIC_BUILDS = [c for c in BUILDS if c[1].operator.kernel_id in (0, 1, 2, 3)] + [
-    ("helm3d", make_build(np.random.default_rng(2).random((2500, 3)), ops.HELMHOLTZ_3D, kappa=3.0, leaf_size=40), 3.0)]
+SYMMETRIC_BUILDS = [c for c in BUILDS if c[1].operator.kernel_id in SYMMETRIC_KERNEL_IDS] + [
+    ("helm3d", make_build(np.random.default_rng(2).random((2500, 3)), ops.HELMHOLTZ_3D, kappa=3.0, leaf_size=40), 3.0),
+    ("hadl", make_build(sphere_points(2000, 3), ops.HELMHOLTZ_ADJOINT_DOUBLE_LAYER_3D, kappa=3.0, leaf_size=50,
+                        geometry=sphere_points(2000, 3)), 3.0)]
 
 
 @pytest.mark.parametrize("case", SYMMETRIC_BUILDS, ids=lambda c: c[0])
