@@ -108,6 +108,31 @@ struct SymSub {
   static constexpr int lanes = 64 / n;  // lanes summing one source's column
 };
 
+// Sum over aligned groups of N = 2, 4, 8 or 16 lanes (every lane of a group gets it), by
+// exchanges inside a row of 16 lanes (DPP): quad permutations [1,0,3,2] and [2,3,0,1], then the
+// half-row mirror (lane i with 7 - i) and the row mirror (i with 15 - i), each pairing lanes of
+// two subgroups that already hold equal partial sums.
+template <int CTRL>
+__device__ inline float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <int N, typename R>
+__device__ inline R group_sum(R v) {
+  static_assert(N == 1 || N == 2 || N == 4 || N == 8 || N == 16, "groups inside one DPP row");
+  if constexpr (N >= 2) v += dpp_mov<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  if constexpr (N >= 4) v += dpp_mov<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  if constexpr (N >= 8) v += dpp_mov<0x141>(v);  // row_half_mirror
+  if constexpr (N >= 16) v += dpp_mov<0x140>(v); // row_mirror
+  return v;
+}
+
 // Orders one wave's LDS accesses around its reverse table: LDS instructions of a wave execute
 // in order, so only the compiler must not move accesses across this point (no wait on the
 // global atomics in flight).
@@ -252,8 +277,7 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
         R v = 0;
 #pragma unroll
         for (int i = 0; i < PER; ++i) v += rvl[((wave * SUB + j) * CW + c) * 64 + pp * PER + i];
-#pragma unroll
-        for (int o = 1; o < LPS; o <<= 1) v += __shfl_xor(v, o, 64);
+        v = group_sum<LPS>(v);
         if (pp == 0 && sub + j < cnt) dst(wb + sub + j, c, double(v));
       }
       wave_sync_lds();
