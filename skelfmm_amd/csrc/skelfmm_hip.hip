@@ -112,15 +112,16 @@ struct SymSub {
 // exchanges inside a row of 16 lanes (DPP): quad permutations [1,0,3,2] and [2,3,0,1], then the
 // half-row mirror (lane i with 7 - i) and the row mirror (i with 15 - i), each pairing lanes of
 // two subgroups that already hold equal partial sums.
-template <int CTRL>
+// (rows outside ROWS receive 0)
+template <int CTRL, int ROWS = 0xF>
 __device__ inline float dpp_mov(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROWS, 0xF, false));
 }
-template <int CTRL>
+template <int CTRL, int ROWS = 0xF>
 __device__ inline double dpp_mov(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, ROWS, 0xF, false);
   return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 template <int N, typename R>
@@ -131,6 +132,24 @@ __device__ inline R group_sum(R v) {
   if constexpr (N >= 8) v += dpp_mov<0x141>(v);  // row_half_mirror
   if constexpr (N >= 16) v += dpp_mov<0x140>(v); // row_mirror
   return v;
+}
+
+// Sum over the whole wave, returned to every lane: each row of 16 as group_sum, then lane 15 of
+// rows 0 and 2 added into rows 1 and 3 (row_bcast:15) and lane 31 into rows 2 and 3
+// (row_bcast:31), so lane 63 holds the total, read out as a wave-uniform value.
+__device__ inline float lane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+__device__ inline double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(int(b), 63);
+  const int hi = __builtin_amdgcn_readlane(int(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+template <typename R>
+__device__ inline R wave_sum(R v) {
+  v = group_sum<16>(v);
+  v += dpp_mov<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_mov<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  return lane63(v);
 }
 
 // Orders one wave's LDS accesses around its reverse table: LDS instructions of a wave execute
@@ -1696,10 +1715,8 @@ __global__ __launch_bounds__(256) void up_gemv_kernel(const R* __restrict__ T, c
       }
 #pragma unroll
       for (int c = 0; c < kIB; ++c) {
-        for (int o = 32; o > 0; o >>= 1) {
-          ar[c] += __shfl_xor(ar[c], o, 64);
-          if (CX) ai[c] += __shfl_xor(ai[c], o, 64);
-        }
+        ar[c] = wave_sum(ar[c]);
+        if (CX) ai[c] = wave_sum(ai[c]);
       }
       if (lane == 0) {
         const R* x = qvec + (long long)(off + j) * w;
@@ -1755,10 +1772,8 @@ __global__ __launch_bounds__(256) void up_gemv1_kernel(const R* __restrict__ T, 
   }
 #pragma unroll
   for (int rr = 0; rr < kRowsPerWave; ++rr) {
-    for (int o = 32; o > 0; o >>= 1) {
-      ar[rr] += __shfl_xor(ar[rr], o, 64);
-      if (CX) ai[rr] += __shfl_xor(ai[rr], o, 64);
-    }
+    ar[rr] = wave_sum(ar[rr]);
+    if (CX) ai[rr] = wave_sum(ai[rr]);
   }
   if (lane < kRowsPerWave) {
     // lane rr writes row j0 + rr (static register selection below)
