@@ -9,7 +9,9 @@ those limits, for both forms and for widths that take each staging path (64 and 
 LDS-direct loads; 17 FP64 / 18 FP32: register staging), against a numpy FP64 sum of the same
 Laplace 2D kernel (0.5 log r^2). The 3D Laplace cases (1/r) run the symmetric kernel in both
 forms and at each inner-loop unroll on groups of up to 600 targets (several passes and work
-items, partial lanes) against entries of several points.
+items, partial lanes) against entries of several points; the 3D Laplace double layer and its
+adjoint run it the same way (both directions' normal projections, FP64 rows in the precision
+form) against the oracle's kernel matrices.
 """
 
 import ctypes
@@ -37,7 +39,8 @@ def _case(seed=0, ne=NE):
     return X, centers, box, sign
 
 
-def _run(precision, width, X, centers, box, sign, q, symmetric=False, kernel=0, nt=NT, e_n=None, sym_unroll=0):
+def _run(precision, width, X, centers, box, sign, q, symmetric=False, kernel=0, nt=NT, e_n=None, sym_unroll=0,
+         normals=None):
     NE = sign.shape[0]
     dim = X.shape[1]
     e_n = np.ones(NE, np.int64) if e_n is None else np.asarray(e_n, np.int64)
@@ -48,6 +51,12 @@ def _run(precision, width, X, centers, box, sign, q, symmetric=False, kernel=0, 
     px = torch.as_tensor(P[:, 0], dtype=real, device=dev)
     py = torch.as_tensor(P[:, 1], dtype=real, device=dev)
     pz = torch.as_tensor(P[:, 2], dtype=real, device=dev) if dim == 3 else None
+    geo = []
+    if normals is not None:  # double layers: normals per point; the precision form's FP64 rows
+        geo = [torch.as_tensor(normals[:, c], dtype=real, device=dev) for c in range(3)]
+        if precision == 1:
+            rows = np.concatenate([X.T, normals.T, np.sum(X * normals, axis=1)[None]])
+            geo.append(torch.as_tensor(np.ascontiguousarray(rows), dtype=torch.float64, device=dev))
     qd = torch.as_tensor(q, dtype=real, device=dev).contiguous()
     ud = torch.zeros_like(qd)
     cen = torch.as_tensor(centers, dtype=torch.float64, device=dev)
@@ -76,6 +85,10 @@ def _run(precision, width, X, centers, box, sign, q, symmetric=False, kernel=0, 
     a.kernel, a.precision, a.width, a.kappa = kernel, precision, width, 0.0
     a.px, a.py, a.pz = hip.ptr(px), hip.ptr(py), (hip.ptr(pz) if pz is not None else None)
     a.sym_unroll = sym_unroll
+    if normals is not None:
+        a.nx, a.ny, a.nz = (hip.ptr(t) for t in geo[:3])
+        if precision == 1:
+            a.geo64, a.geo64_n = hip.ptr(geo[3]), int(X.shape[0])
     a.centers = hip.ptr(cen) if precision == 1 else None
     a.q, a.u = hip.ptr(qd), hip.ptr(ud)
     a.ngroups, a.nentries, a.nitems = 1, NE, 1
@@ -151,6 +164,42 @@ def test_symmetric_laplace3d_unrolls(precision, sym_unroll, nt, ne):
     print(f"3D symmetric precision {precision} unroll {sym_unroll}: {nt} targets, {ne} entries, "
           f"{X.shape[0] - nt} sources", flush=True)
     u = _run(precision, 1, X, centers, box, sign, q, symmetric=True, kernel=1, nt=nt, e_n=e_n, sym_unroll=sym_unroll)
+    err_t = np.linalg.norm(u[:nt] - ref[:nt]) / np.linalg.norm(ref[:nt])
+    err_s = np.linalg.norm(u[nt:] - ref[nt:]) / np.linalg.norm(ref[nt:])
+    print(f"  rel-l2 targets {err_t:.2e}, sources {err_s:.2e}", flush=True)
+    bar = 1e-13 if precision == 0 else 2e-6
+    assert err_t < bar and err_s < bar
+
+
+@pytest.mark.parametrize("kernel,name", [(4, "laplace_double_layer_3d"), (5, "laplace_adjoint_double_layer_3d")])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("nt,ne", [(40, 100), (97, 300), (600, 30)])
+def test_symmetric_double_layers(kernel, name, precision, nt, ne):
+    """Double-layer pairs evaluated once for both directions: forward K(t, s), reverse K(s, t)
+    (the normal on the other side), against the oracle's kernel matrices in FP64; 300 entries
+    take the entry-by-entry staging, 100 the stream across entries (FP32 form)."""
+    import torch
+
+    from oracle.kernels import GEOM_SIDE, kernel_matrix_torch
+
+    X, centers, box, sign, e_n = _case3(nt, ne, seed=5)
+    nrm = np.random.default_rng(6).normal(size=X.shape)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    q = np.random.default_rng(7).normal(size=(X.shape[0], 1))
+    sgn = np.repeat(sign, e_n)
+    T, S = torch.as_tensor(X[:nt])[None], torch.as_tensor(X[nt:])[None]
+    NT_, NS_ = torch.as_tensor(nrm[:nt])[None], torch.as_tensor(nrm[nt:])[None]
+    if GEOM_SIDE[name] == "source":
+        Kf = kernel_matrix_torch(name, T, S, 0.0, False, geom_src=NS_)[0].numpy()
+        Kr = kernel_matrix_torch(name, S, T, 0.0, False, geom_src=NT_)[0].numpy()
+    else:
+        Kf = kernel_matrix_torch(name, T, S, 0.0, False, geom_trg=NT_)[0].numpy()
+        Kr = kernel_matrix_torch(name, S, T, 0.0, False, geom_trg=NS_)[0].numpy()
+    ref = np.zeros_like(q)
+    ref[:nt] = (Kf * sgn[None, :]) @ q[nt:]
+    ref[nt:] = (Kr * sgn[:, None]) @ q[:nt]
+    print(f"{name} symmetric precision {precision}: {nt} targets, {ne} entries", flush=True)
+    u = _run(precision, 1, X, centers, box, sign, q, symmetric=True, kernel=kernel, nt=nt, e_n=e_n, normals=nrm)
     err_t = np.linalg.norm(u[:nt] - ref[:nt]) / np.linalg.norm(ref[:nt])
     err_s = np.linalg.norm(u[nt:] - ref[nt:]) / np.linalg.norm(ref[nt:])
     print(f"  rel-l2 targets {err_t:.2e}, sources {err_s:.2e}", flush=True)
