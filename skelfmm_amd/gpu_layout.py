@@ -10,7 +10,7 @@ Every array equals the host statement's bit for bit (``tests/test_gpu_tree.py``)
   list, rows with an empty target or source set dropped;
 * the symmetric split (kernels of ``layout.SYMMETRIC_KERNEL_IDS``): a row pairs with its mirror when both are unique
   and each charge kind equals its source set's kind; the representative is the row whose target
-  set orders first;
+  set pads less to the symmetric kernel's 64-lane rows (``layout.pair_representative``);
 * per table: rows stably sorted by (target box, target kind, source box), one group per target
   set, one entry per row, work items of up to ``tpi`` targets and a source-run window of
   ``max_sources`` points, heaviest first (stable).
@@ -21,7 +21,8 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from .layout import (SRC_NEG, SRC_SELF, SYM_TARGETS_PER_ITEM, SYMMETRIC_KERNEL_IDS, TARGETS_PER_ITEM, NearLists)
+from .layout import (SRC_NEG, SRC_SELF, SYM_TARGETS_PER_ITEM, SYMMETRIC_KERNEL_IDS, TARGETS_PER_ITEM, NearLists,
+                     pair_representative)
 
 __all__ = ["near_field_tables"]
 
@@ -106,7 +107,7 @@ def _tables(R, off, bs, rk, nslots, max_sources, tpi, nboxes):
     return NL
 
 
-def _pair_rows(R, nboxes):
+def _pair_rows(R, nboxes, sizes=None):
     """(one-sided rows, pair representatives) as ``layout.pair_rows``."""
     if R.shape[0] == 0:
         return R, R
@@ -120,7 +121,7 @@ def _pair_rows(R, nboxes):
     single = torch.isin(key, uk[cnt == 1])
     ok = single & (R[:, 0] != R[:, 2]) & (R[:, 4] == R[:, 3])
     paired = ok & torch.isin(pkey, key[ok])
-    rep = paired & (a < b)
+    rep = paired & pair_representative(a, b, sizes)
     return R[~paired], R[rep]
 
 
@@ -146,11 +147,12 @@ def near_field_tables(fmm, off, bs, rk, nslots, max_sources, device):
     R = torch.cat(parts) if parts else torch.zeros((0, 6), dtype=torch.int64, device=dev)
     tcount = torch.where(R[:, 1] == 0, bs_d[R[:, 0]], rk_d[R[:, 0]])
     scount = torch.where(R[:, 3] == 0, bs_d[R[:, 2]], rk_d[R[:, 2]])
-    R = R[(tcount > 0) & (scount > 0)]
+    keep = (tcount > 0) & (scount > 0)
+    R, tcount, scount = R[keep], tcount[keep], scount[keep]
     full = _tables(R, off_d, bs_d, rk_d, nslots, max_sources, TARGETS_PER_ITEM, nb)
     split = None
     if fmm.operator.kernel_id in SYMMETRIC_KERNEL_IDS:
-        one_rows, sym_rows = _pair_rows(R, nb)
+        one_rows, sym_rows = _pair_rows(R, nb, sizes=(tcount, scount))
         if sym_rows.shape[0]:
             one = _tables(one_rows, off_d, bs_d, rk_d, nslots, max_sources, TARGETS_PER_ITEM, nb)
             sym = _tables(sym_rows, off_d, bs_d, rk_d, nslots, max_sources, SYM_TARGETS_PER_ITEM, nb)

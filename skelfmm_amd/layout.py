@@ -512,12 +512,32 @@ def _lists_from_rows(R, off, bs, rk, nslots, max_sources, tpi):
     return NL
 
 
-def pair_rows(R, nboxes):
+# the symmetric kernel takes a work item's targets in rows of this many lanes
+SYM_LANES = 64
+
+
+def pair_representative(a, b, sizes):
+    """Which row of a pair is its representative (array of bools over the rows).
+
+    Without ``sizes``: the row whose target set orders first (``a < b``). With ``sizes`` (per row,
+    the target and source set sizes): the row whose target set leaves fewer idle lanes times
+    sources in the symmetric kernel's rows of ``SYM_LANES`` targets, ties by order. The rule is
+    antisymmetric, so exactly one row of every pair is chosen; host and device builds share it.
+    """
+    if sizes is None:
+        return a < b
+    nt, ns = sizes
+    wt = ((nt + SYM_LANES - 1) // SYM_LANES * SYM_LANES - nt) * ns  # this row as targets
+    ws = ((ns + SYM_LANES - 1) // SYM_LANES * SYM_LANES - ns) * nt  # the mirror row as targets
+    return (wt < ws) | ((wt == ws) & (a < b))
+
+
+def pair_rows(R, nboxes, sizes=None):
     """Split rows into (one-sided rows, pair representatives).
 
     Row (U, a, V, b, q, neg) pairs with (V, b, U, a, q', neg) for U != V when both are unique
     and each charge kind equals its source set's kind (a set's charges and potentials then share
-    slots). The representative is the row whose target set orders first.
+    slots). The representative follows ``pair_representative``.
     """
     if R.shape[0] == 0:
         return R, R
@@ -531,7 +551,7 @@ def pair_rows(R, nboxes):
     ok = single & (R[:, 0] != R[:, 2]) & (R[:, 4] == R[:, 3])
     okkeys = key[ok]
     paired = ok & np.isin(pkey, okkeys)
-    rep = paired & (a < b)
+    rep = paired & pair_representative(a, b, sizes)
     return R[~paired], R[rep]
 
 
@@ -565,7 +585,8 @@ def _near_field(L, fmm, off, bs, rk, max_sources):
 
     tcount = np.where(R[:, 1] == 0, bs[R[:, 0]], rk[R[:, 0]])
     scount = np.where(R[:, 3] == 0, bs[R[:, 2]], rk[R[:, 2]])
-    R = R[(tcount > 0) & (scount > 0)]
+    keep = (tcount > 0) & (scount > 0)
+    R, tcount, scount = R[keep], tcount[keep].astype(np.int64), scount[keep].astype(np.int64)
 
     full = _lists_from_rows(R, off, bs, rk, nslots, max_sources, TARGETS_PER_ITEM)
     for k in NearLists.FIELDS + ("items", "item_cost", "interactions"):
@@ -573,7 +594,7 @@ def _near_field(L, fmm, off, bs, rk, max_sources):
     L.evaluations = L.interactions
     L.split = None
     if fmm.operator.kernel_id in SYMMETRIC_KERNEL_IDS:
-        one_rows, sym_rows = pair_rows(R, fmm.nboxes)
+        one_rows, sym_rows = pair_rows(R, fmm.nboxes, sizes=(tcount, scount))
         if sym_rows.shape[0]:
             one = _lists_from_rows(one_rows, off, bs, rk, nslots, max_sources, TARGETS_PER_ITEM)
             sym = _lists_from_rows(sym_rows, off, bs, rk, nslots, max_sources, SYM_TARGETS_PER_ITEM)

@@ -189,6 +189,37 @@ def test_pair_rows_partition():
     assert {tuple(r) for r in one} == {tuple(r) for r in R[6:]}
 
 
+def test_pair_orientation_pads_less():
+    """With set sizes, each pair's representative puts the set that leaves fewer idle lanes
+    (times sources) in the symmetric kernel's 64-lane rows on the target side: the rows still
+    split into one-sided rows plus one representative per pair, and the padded evaluations never
+    exceed those of the order-based choice."""
+    from skelfmm_amd.layout import SYM_LANES, pair_rows
+
+    rng = np.random.default_rng(3)
+    nsets = 400
+    size = rng.integers(1, 600, size=nsets)
+    A = rng.integers(0, nsets, size=3000)
+    B = rng.integers(0, nsets, size=3000)
+    keep = A != B
+    A, B = A[keep], B[keep]
+    pairs = np.unique(np.stack([np.minimum(A, B), np.maximum(A, B)], 1), axis=0)
+    rows = np.concatenate([pairs, pairs[:, ::-1]])
+    # sets are boxes of kind 0 (box = set index); charge kind 0, sign +
+    R = np.stack([rows[:, 0], np.zeros(len(rows), np.int64), rows[:, 1], np.zeros(len(rows), np.int64),
+                  np.zeros(len(rows), np.int64), np.zeros(len(rows), np.int64)], 1)
+    nt, ns = size[R[:, 0]], size[R[:, 2]]
+    one0, rep0 = pair_rows(R, nsets)
+    one1, rep1 = pair_rows(R, nsets, sizes=(nt, ns))
+    assert one0.shape[0] == one1.shape[0] == 0 and rep0.shape[0] == rep1.shape[0] == pairs.shape[0]
+    got = {tuple(sorted((r[0], r[2]))) for r in rep1}
+    assert got == {tuple(p) for p in pairs}  # one representative per pair
+    pad = lambda n: (n + SYM_LANES - 1) // SYM_LANES * SYM_LANES
+    waste = lambda rep: int(np.sum((pad(size[rep[:, 0]]) - size[rep[:, 0]]) * size[rep[:, 2]]))
+    assert waste(rep1) <= waste(rep0)
+    assert waste(rep1) < waste(rep0)  # sizes this random always leave room
+
+
 def test_merged_layout_split_emulates_every_instance():
     from skelfmm_amd.layout import merge_layouts
 
