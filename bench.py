@@ -288,6 +288,44 @@ def run_annuli(args, steps, warmup, with_cpu):
         errs64.append(float(np.linalg.norm(a64[idx, :4].cpu().numpy() - ref) / np.linalg.norm(ref)))
     log(f"annuli rel-l2 vs direct sum (max over instances): FP32 {max(errs32):.3e}, FP64 {max(errs64):.3e}")
 
+    # the same instances built on the GPU (tree, batched adaptive IDs, device work tables)
+    gpu_build = None
+    if not args.no_gpu_build:
+        from skelfmm_amd.gpu_tree import GPUBalancedTree
+
+        # first-call costs (rocPRIM, the device-table kernels) outside the timing
+        wt = GPUBalancedTree(pts[0][:20000], leaf_size=leaf)
+        wf = RecursiveSkeletonization(wt, op, kappa=0.0, tol=tol)
+        wf.skel_tree(npoints_max=500, device="gpu")
+        wf.setup_lists()
+        BatchedInstances([wf], use_single_precision=True)
+        del wt, wf
+        torch.cuda.synchronize()
+        tg = time.perf_counter()
+        gbuilds = []
+        for P in pts:
+            gt = GPUBalancedTree(P, leaf_size=leaf)
+            gf = RecursiveSkeletonization(gt, op, kappa=0.0, tol=tol)
+            gf.skel_tree(npoints_max=500, device="gpu")
+            gf.setup_lists()
+            gbuilds.append(gf)
+        dg = BatchedInstances(gbuilds, use_single_precision=True)
+        torch.cuda.synchronize()
+        t_g = time.perf_counter() - tg
+        yg = dg.matvec(x32)
+        errs_g = []
+        for i, (P, q, a) in enumerate(zip(pts, qs, dg.split(yg))):
+            idx = np.random.default_rng(i).choice(P.shape[0], min(100, P.shape[0]), replace=False)
+            ref = direct_sum_c(op.name, P[idx], P, q[:, :4].astype(np.float64))
+            errs_g.append(float(np.linalg.norm(a[idx, :4].cpu().numpy().astype(np.float64) - ref) / np.linalg.norm(ref)))
+        gpu_build = {"build_s": round(t_g, 2), "host_build_s": round(build_s, 2),
+                     "rank_sum": int(sum(int(b.rank_list.sum()) for b in gbuilds)),
+                     "host_rank_sum": int(sum(int(b.rank_list.sum()) for b in builds)),
+                     "fp32_max_rel_l2_err": max(errs_g)}
+        log(f"annuli GPU build: {t_g:.2f}s (host {build_s:.1f}s), ranks {gpu_build['rank_sum']} "
+            f"(host {gpu_build['host_rank_sum']}), FP32 apply error {max(errs_g):.3e}")
+        del dg, yg, gbuilds
+
     mfma_tf = 2.0 * d32.interactions * width / p32 / 1e12
     out = {
         "value": round(ntot * width / t32 / 1e6, 3), "unit": "Mpoints*RHS/s", "ms_per_step": round(t32 * 1e3, 4),
@@ -300,6 +338,7 @@ def run_annuli(args, steps, warmup, with_cpu):
                  "max_rel_l2_err": max(errs64)},
         "fp32_max_rel_l2_err": max(errs32),
         "build_s": round(build_s, 2),
+        "gpu_build": gpu_build,
         "roofline": {"bound": "mfma", "achieved": round(mfma_tf, 3), "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
                      "frac": round(mfma_tf / PEAK_TFLOPS["fp32"], 4),
                      "kernel": "near field (matrix-core kernel at >= 16 RHS unless the autotuner picked VALU)",

@@ -626,7 +626,11 @@ class BatchedInstances(SkelFMMBatched):
         self.generation = 0
         self.instance_sizes = [int(b.N) for b in builds]
         self.ninstances = len(builds)
-        L = merge_layouts([build_layout(b, max_sources=max_sources) for b in builds])
+        nf_dev = None
+        if all(getattr(b, "build_device", "cpu") == "gpu" for b in builds):  # GPU builds: tables there too
+            hip.require_gpu()
+            nf_dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+        L = merge_layouts([build_layout(b, max_sources=max_sources, near_field_device=nf_dev) for b in builds])
         self.upload_layout(L, use_single_precision, device)
 
     def concat(self, xs):

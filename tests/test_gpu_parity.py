@@ -279,6 +279,30 @@ def test_batched_instances_match_separate_applies():
         assert rel_l2(part.cpu().numpy(), ref) <= FP64_TOL
 
 
+def test_batched_instances_of_gpu_builds():
+    """Instances built on the GPU (tree, batched IDs, device work tables) and merged: each
+    instance's part of the one apply matches the oracle on that instance's build data."""
+    from skelfmm_amd.batched import BatchedInstances
+    from skelfmm_amd.build import RecursiveSkeletonization
+    from skelfmm_amd.gpu_tree import GPUBalancedTree
+
+    pts = _annulus_instances((3000, 9000, 1200, 5000))
+    builds = []
+    for P in pts:
+        f = RecursiveSkeletonization(GPUBalancedTree(P, leaf_size=64), ops.LAPLACE_2D, kappa=0.0, tol=1e-5)
+        f.skel_tree(npoints_max=500, device="gpu")
+        f.setup_lists()
+        builds.append(f)
+    rng = np.random.default_rng(0)
+    qs = [rng.normal(size=(P.shape[0], 3)) for P in pts]
+    dev = BatchedInstances(builds)
+    y = dev.matvec(dev.concat(qs))
+    torch.cuda.synchronize()
+    for part, b, q in zip(dev.split(y), builds, qs):
+        ref = BatchedApplyCPU(b, bytes_budget=2e8).matvec(q).numpy()
+        assert rel_l2(part.cpu().numpy(), ref) <= FP64_TOL
+
+
 @pytest.mark.parametrize("name,nrhs", [("laplace_2d", 64), ("laplace_3d", 20), ("laplace_dl", 16), ("laplace_2d", 18),
                                        ("laplace_3d", 100), ("laplace_adl", 16)])
 def test_mfma_multi_rhs_precision_form(name, nrhs):
