@@ -79,13 +79,15 @@ enum skf_status {
  * vectors as 2*nrhs real channels.
  */
 /*
- * Split near-field work for the symmetric single-layer kernels (Laplace / Helmholtz, 2D and 3D):
- * the same group / entry / item tables as below, once for the pairs of lists that interact
- * in both directions ("sym": one representative entry per pair, items of up to 512 targets)
- * and once for the rest ("one": self segments and unpaired entries, items of up to 64 targets).
- * A sym entry s of group g adds K(t, s) q[e_q + j] into u[g_out + i] and K(t, s) q[g_out + i]
- * into u[e_q + j]: the reverse output of an entry is its charge offset (a set's charges and
- * potentials share slots and kind).
+ * Split near-field work for the kernels evaluated once per point pair (the single layers,
+ * Laplace / Helmholtz in 2D and 3D, and the 3D double layers): the same group / entry / item
+ * tables as below, once for the pairs of lists that interact in both directions ("sym": one
+ * representative entry per pair, items of up to 512 targets) and once for the rest ("one": self
+ * segments and unpaired entries, items of up to 64 targets). A sym entry s of group g adds
+ * K(t, s) q[e_q + j] into u[g_out + i] and K(s, t) q[g_out + i] into u[e_q + j] (K(s, t) =
+ * K(t, s) for the single layers; for the double layers the same g(|t - s|) with the other
+ * point's normal projection): the reverse output of an entry is its charge offset (a set's
+ * charges and potentials share slots and kind).
  */
 typedef struct skf_p2p_lists {
   int ngroups;
