@@ -55,9 +55,10 @@ HBM_BYTES_PER_S = 8.0e12
 # HBM bytes per launch of the FP32 symmetric kernel on the headline workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.00, WRITE_SIZE x 1.00: the factors a 512 MiB copy in the
 # same run reads), keyed by (N, tol, leaf), with the setting that was profiled (symmetric unroll,
-# source-run length; the traffic barely depends on either); profiles/apply_laplace3d_uniform1M_hbm_traffic.md
-P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400): {"bytes": 584_339_456 + 635_866_112, "sym_unroll": 2,
-                                                   "max_sources": 4096}}
+# source-run length, self block; the traffic barely depends on any of them);
+# profiles/apply_laplace3d_uniform1M_hbm_traffic.md
+P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400): {"bytes": 568_321_905 + 640_487_547, "sym_unroll": 2,
+                                                   "max_sources": 4096, "self_block": 192}}
 
 
 def log(msg):
@@ -174,8 +175,8 @@ def _tuned_summary(t):
     """The autotuner's choice without its per-candidate timings."""
     if not t:
         return t
-    out = {k: t[k] for k in ("max_sources", "sym_unroll", "mfma_min_channels", "item_order", "near_field_ms", "cached",
-                             "pinned")
+    out = {k: t[k] for k in ("max_sources", "sym_unroll", "self_block", "mfma_min_channels", "item_order",
+                             "near_field_ms", "cached", "pinned")
            if k in t}
     lv = t.get("levels")
     if lv:
@@ -560,13 +561,16 @@ def run_cube(args):
         f"interactions/apply {d64.interactions:.3e}")
 
     tuned = None
-    if args.max_sources or args.sym_unroll:  # a pinned setting (profiling runs)
+    if args.max_sources or args.sym_unroll or args.self_block:  # a pinned setting (profiling runs)
         for d in (d64, d32):
             if args.max_sources:
                 d.set_near_field_split(args.max_sources)
             if args.sym_unroll:
                 d.set_sym_unroll(args.sym_unroll)
-        tuned = {k: {"max_sources": int(d.layout.max_sources), "sym_unroll": int(d.sym_unroll or 2), "pinned": True}
+            if args.self_block:
+                d.set_self_block(args.self_block)
+        tuned = {k: {"max_sources": int(d.layout.max_sources), "sym_unroll": int(d.sym_unroll or 2),
+                     "self_block": int(d.self_block), "pinned": True}
                  for k, d in (("fp64", d64), ("fp32", d32))}
     elif not args.no_tune:
         from skelfmm_amd import autotune
@@ -658,7 +662,7 @@ def run_cube(args):
     k_ms = s32 if sym else p32
     ach = served * fpi / k_ms / 1e12
     executed = (d32.sym_pairs * (fpi + 2) if sym else d32.interactions * fpi) / k_ms / 1e12
-    setting = (int(d32.sym_unroll or 2), int(d32.layout.max_sources))
+    setting = (int(d32.sym_unroll or 2), int(d32.layout.max_sources), int(d32.self_block))
     profiled = P2P_FP32_TRAFFIC_BYTES.get((N, args.tol, args.leaf)) if sym else None
     roofline = {
         "bound": "valu",
@@ -669,8 +673,9 @@ def run_cube(args):
         "traffic": profiled["bytes"] if profiled else None,
         "traffic_unit": "bytes per launch (HBM read + write) of the dominant kernel, rocprofv3 FETCH_SIZE / "
                         "WRITE_SIZE scaled by a 512 MiB copy",
-        "traffic_profiled_setting": None if not profiled else {k: profiled[k] for k in ("sym_unroll", "max_sources")},
-        "run_setting": {"sym_unroll": setting[0], "max_sources": setting[1]},
+        "traffic_profiled_setting": None if not profiled else {k: profiled[k] for k in ("sym_unroll", "max_sources",
+                                                                                        "self_block")},
+        "run_setting": {"sym_unroll": setting[0], "max_sources": setting[1], "self_block": setting[2]},
         "kernel": ("p2p_sym_kernel<float, LAPLACE_3D> (pairs of lists, both directions; zero fill and fold of the "
                    "reverse sums included)" if sym else "p2p_kernel<float, LAPLACE_3D>"),
         "kernel_ms": round(k_ms * 1e3, 4),
@@ -744,6 +749,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="skip the near-field autotuner (on by default)")
     ap.add_argument("--max-sources", type=int, default=0, help="headline workload: pin the source-run length")
     ap.add_argument("--sym-unroll", type=int, default=0, help="headline workload: pin the symmetric unroll")
+    ap.add_argument("--self-block", type=int, default=0,
+                    help="headline workload: pin the block of the self segments' symmetric evaluation")
     ap.add_argument("--no-gpu-build", action="store_true",
                     help="laplace3d_cube: skip the second build with the IDs on the GPU")
     args = ap.parse_args()

@@ -14,6 +14,8 @@ near-field launch over
 * where the symmetric near-field kernel runs (one channel, single-layer kernel),
   the number of sources per step of its inner loop (1, 2 or 4: instruction-level
   parallelism against registers, i.e. waves per SIMD),
+* with the symmetric kernel in use, the block size of the self segments' symmetric evaluation
+  (``SELF_BLOCKS``, or none: self segments one-sided), timed with the best setting of the above,
 * per tree level and direction, the launch shape of the interpolation stage
   (``tune_levels``: vector loads in flight per lane for one channel, matrix cores or
   VALU GEMM for >= 16 real FP32 channels), each stage timed alone,
@@ -34,6 +36,8 @@ import torch
 from .layout import ITEM_ORDERS
 
 SPLITS = (1024, 2048, 4096, 8192, 16384, 32768, 65536)
+# block sizes of the self segments' symmetric evaluation tried (0, one-sided, is always tried)
+SELF_BLOCKS = (128, 192, 256)
 
 
 def cache_path():
@@ -69,6 +73,8 @@ def apply_setting(dev, setting):
     dev.set_mfma_min_channels(int(setting.get("mfma_min_channels", 0)))
     if hasattr(dev, "set_sym_unroll"):
         dev.set_sym_unroll(int(setting.get("sym_unroll", 0)))
+    if hasattr(dev, "set_self_block"):
+        dev.set_self_block(int(setting.get("self_block", 0)))
     if int(setting["max_sources"]) != int(getattr(dev.layout, "max_sources", -1)):
         dev.set_near_field_split(int(setting["max_sources"]))
     order = setting.get("item_order", "heavy")
@@ -143,6 +149,16 @@ def tune(dev, width=1, reps=3, use_cache=True, splits=SPLITS, levels=True):
     best = {"max_sources": ms, "mfma_min_channels": mf, "sym_unroll": un, "item_order": order,
             "near_field_ms": round(t, 4), "tried": [[round(a, 4), b, c, d, e] for a, b, c, d, e in results]}
     apply_setting(dev, best)
+    if sym and hasattr(dev, "set_self_block"):
+        timed = []
+        for blk in (0,) + tuple(SELF_BLOCKS):
+            dev.set_self_block(blk)
+            timed.append((dev.time_near_field(width, reps=reps), blk))
+        t, blk = min(timed)
+        best["self_block"] = blk
+        best["near_field_ms"] = round(t, 4)
+        best["self_block_tried"] = [[round(a, 4), b] for a, b in timed]
+        apply_setting(dev, best)
     if levels and (len(dev.layout.up) or len(dev.layout.down)):
         best["levels"] = tune_levels(dev, width, reps=reps)
     if use_cache:

@@ -194,25 +194,14 @@ class SkelFMMBatched(torch.nn.Module):
         self.item_order = "heavy"
         self.rows_exclusive = self._rows_exclusive(L)
         self._nf["items"] = self._items_tensor(L)
-        # symmetric split of the near field (single-layer kernels): pair lists + the rest
-        self._split = None
-        if getattr(L, "split", None) is not None:
-            self._split = []
-            for part in L.split:
-                t = {k: i32(getattr(part, k)) for k in
-                     ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags")}
-                t["e_shift"] = self._entry_shifts(part)
-                t["items"] = self._items_tensor(part)
-                self._split.append(t)
-            self._split_structs = [hip.P2PLists(), hip.P2PLists()]
-            self._fill_split_structs()
+        # symmetric split of the near field: pair lists + the one-sided rest (the tables as
+        # uploaded; set_self_block re-derives the split from them)
+        self._split_base = getattr(L, "split", None)
+        self.self_block = 0
+        self._upload_split(L)
         self.use_symmetric = self._split is not None
         self.sym_unroll = 0
         self.interactions = L.interactions
-        self.evaluations = getattr(L, "evaluations", L.interactions)
-        # pairs evaluated by the symmetric kernel / interactions of the one-sided rest
-        self.sym_pairs = L.split[1].interactions if self._split is not None else 0
-        self.one_interactions = L.split[0].interactions if self._split is not None else L.interactions
         self.nslots = L.nslots
         self.N = L.N
         # bound of every pair distance: the diameter of the points' bounding box
@@ -239,6 +228,46 @@ class SkelFMMBatched(torch.nn.Module):
         gbox = np.repeat(np.asarray(lists.g_box, np.int64), np.asarray(lists.g_e1) - np.asarray(lists.g_e0))
         sh = (C[gbox] - C[np.asarray(lists.e_box, np.int64)]).astype(np.float32)
         return torch.as_tensor(np.ascontiguousarray(sh.reshape(-1)), device=self.device)
+
+    def _upload_split(self, L):
+        """Device copies of the split tables of ``L`` (or none) and the interaction counts."""
+        self._split = None
+        if getattr(L, "split", None) is not None:
+            self._split = []
+            for part in L.split:
+                t = {k: torch.as_tensor(np.ascontiguousarray(getattr(part, k), dtype=np.int32), device=self.device)
+                     for k in ("g_pt", "g_n", "g_out", "g_box", "e_pt", "e_n", "e_q", "e_box", "e_flags")}
+                t["e_shift"] = self._entry_shifts(part)
+                t["items"] = self._items_tensor(part)
+                self._split.append(t)
+            self._split_structs = [hip.P2PLists(), hip.P2PLists()]
+            self._fill_split_structs()
+        self.evaluations = getattr(L, "evaluations", L.interactions)
+        # pairs evaluated by the symmetric kernel / interactions of the one-sided rest
+        self.sym_pairs = L.split[1].interactions if self._split is not None else 0
+        self.one_interactions = L.split[0].interactions if self._split is not None else L.interactions
+
+    def set_self_block(self, block):
+        """Self segments (a target set against its own points) evaluated symmetrically in blocks of
+        ``block`` targets: the pairs of different blocks join the symmetric kernel's tables, the
+        diagonal blocks stay one-sided (``layout.split_self_blocks``). 0 (the default) keeps every
+        self segment one-sided. Needs the symmetric split; the results agree to rounding."""
+        from .layout import make_items, split_self_blocks
+
+        block = int(block)
+        if block < 0:
+            raise ValueError(f"self block must be >= 0, got {block}")
+        if self._split_base is None or block == self.self_block:
+            return
+        L = self.layout
+        one, sym = split_self_blocks(*self._split_base, L.max_sources, block)
+        for part in (one, sym):  # work items for the current source-run length
+            part.items, part.item_cost = make_items(part.g_n, part.g_e0, part.g_e1, part.e_n, L.max_sources, part.tpi)
+        L.split = (one, sym)
+        L.evaluations = one.interactions + sym.interactions
+        self.self_block = block
+        self._upload_split(L)
+        self._invalidate()
 
     def _fill_split_structs(self):
         for st, t in zip(self._split_structs, self._split):

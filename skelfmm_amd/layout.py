@@ -555,6 +555,88 @@ def pair_rows(R, nboxes, sizes=None):
     return R[~paired], R[rep]
 
 
+_GROUP_FIELDS = ("g_pt", "g_n", "g_out", "g_box", "g_kind")
+_ENTRY_FIELDS = ("e_pt", "e_n", "e_q", "e_qkind", "e_box", "e_flags")
+
+
+def _assemble(groups, counts, entries, max_sources, tpi):
+    """NearLists from group columns, each group's entry count and the entry columns (in group order)."""
+    NL = NearLists()
+    NL.tpi = int(tpi)
+    for k in _GROUP_FIELDS:
+        setattr(NL, k, np.asarray(groups[k], dtype=np.int32))
+    for k in _ENTRY_FIELDS:
+        setattr(NL, k, np.asarray(entries[k], dtype=np.int32))
+    counts = np.asarray(counts, dtype=np.int64)
+    NL.g_e1 = np.cumsum(counts).astype(np.int64)
+    NL.g_e0 = NL.g_e1 - counts
+    NL.items, NL.item_cost = make_items(NL.g_n, NL.g_e0, NL.g_e1, NL.e_n, max_sources, tpi)
+    tn = np.repeat(NL.g_n.astype(np.int64), counts)
+    NL.interactions = int(np.sum(tn * NL.e_n.astype(np.int64)))
+    return NL
+
+
+def split_self_blocks(one, sym, max_sources, block):
+    """Symmetric evaluation inside the self segments: (one, sym) with every self segment (a
+    target set against its own points, one-sided with SRC_SELF) of more than ``block`` points cut
+    into blocks of ``block`` targets. The pairs of two different blocks move to the symmetric
+    table -- block i as targets against the points after it, one group per block, evaluated once
+    for both directions -- and each diagonal block stays one-sided as a group of its own (self
+    flag kept, so i == j is skipped). A segment qualifies when its charge slots are its potential
+    slots. Same interactions: n^2 = sum of diagonal blocks + 2 x the pairs of different blocks.
+    ``block`` 0 returns (one, sym) unchanged; multiples of ``SYM_LANES`` leave no idle lanes in
+    the symmetric kernel's rows."""
+    block = int(block)
+    if block <= 0:
+        return one, sym
+    col = lambda P, k: np.asarray(getattr(P, k), dtype=np.int64)
+    g = {k: col(one, k) for k in _GROUP_FIELDS}
+    e = {k: col(one, k) for k in _ENTRY_FIELDS}
+    ne = col(one, "g_e1") - col(one, "g_e0")
+    eg = np.repeat(np.arange(ne.shape[0], dtype=np.int64), ne)  # group of every entry
+    sel = (((e["e_flags"] & SRC_SELF) != 0) & (e["e_n"] > block) & (e["e_pt"] == g["g_pt"][eg])
+           & (e["e_n"] == g["g_n"][eg]) & (e["e_q"] == g["g_out"][eg]))
+    if not sel.any():
+        return one, sym
+    s = np.nonzero(sel)[0]
+    n = e["e_n"][s]
+    # diagonal blocks: one group and one self entry each
+    nb = (n + block - 1) // block
+    ds = np.repeat(s, nb)
+    d0 = _ranges(nb) * block
+    dn = np.minimum(block, np.repeat(n, nb) - d0)
+    # pairs of different blocks: block i (full) against the points after it
+    ps = np.repeat(s, nb - 1)
+    p0 = _ranges(nb - 1) * block
+    pn = np.repeat(n, nb - 1) - p0 - block
+
+    def sub_groups(ent, start, cnt):
+        gi = eg[ent]
+        return {"g_pt": g["g_pt"][gi] + start, "g_n": cnt, "g_out": g["g_out"][gi] + start,
+                "g_box": g["g_box"][gi], "g_kind": g["g_kind"][gi]}
+
+    def sub_entries(ent, start, cnt, flags):
+        return {"e_pt": e["e_pt"][ent] + start, "e_n": cnt, "e_q": e["e_q"][ent] + start,
+                "e_qkind": e["e_qkind"][ent], "e_box": e["e_box"][ent], "e_flags": flags}
+
+    keep = ~sel
+    kept_ne = np.bincount(eg[keep], minlength=ne.shape[0])
+    live = kept_ne > 0
+    dg = sub_groups(ds, d0, dn)
+    de = sub_entries(ds, d0, dn, e["e_flags"][ds])
+    one2 = _assemble({k: np.concatenate((g[k][live], dg[k])) for k in _GROUP_FIELDS},
+                     np.concatenate((kept_ne[live], np.ones(ds.shape[0], np.int64))),
+                     {k: np.concatenate((e[k][keep], de[k])) for k in _ENTRY_FIELDS}, max_sources, one.tpi)
+    sg = sub_groups(ps, p0, np.full(ps.shape[0], block, np.int64))
+    se = sub_entries(ps, p0 + block, pn, e["e_flags"][ps] & ~SRC_SELF)
+    sym_ne = col(sym, "g_e1") - col(sym, "g_e0")
+    sym2 = _assemble({k: np.concatenate((col(sym, k), sg[k])) for k in _GROUP_FIELDS},
+                     np.concatenate((sym_ne, np.ones(ps.shape[0], np.int64))),
+                     {k: np.concatenate((col(sym, k), se[k])) for k in _ENTRY_FIELDS}, max_sources, sym.tpi)
+    assert one2.interactions + 2 * sym2.interactions == one.interactions + 2 * sym.interactions
+    return one2, sym2
+
+
 def _near_field(L, fmm, off, bs, rk, max_sources):
     nslots = L.nslots
     # (target box, target kind) -> list of (source box, source kind, charge kind, flags); kind 0 = all, 1 = skel

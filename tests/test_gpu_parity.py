@@ -544,6 +544,48 @@ def test_symmetric_near_field_matches_one_sided(name, single):
             assert rel_l2(y_sym, ref) <= FP64_TOL
 
 
+@pytest.mark.parametrize("name,block", [("laplace_3d_cube", 64), ("laplace_3d_cube", 32), ("helmholtz_2d_square", 32),
+                                        ("helmholtz_dl_sphere", 64), ("laplace_adl_sphere", 48)])
+@pytest.mark.parametrize("single", [False, True])
+def test_self_blocks_match_one_sided_self(name, block, single):
+    """Self segments evaluated symmetrically in blocks (pairs of different blocks in the symmetric
+    kernel, diagonal blocks one-sided) against one-sided self segments: FP64 to 1e-13, FP32 form
+    to 3e-6; back to block 0 gives the first result again; the autotuner's self-block stage
+    keeps parity."""
+    from skelfmm_amd import autotune
+    from skelfmm_amd.batched import SkelFMMBatched
+
+    X, op, kappa, geom, leaf = _case(name)
+    print(f"case {name} block={block} single={single} N={X.shape[0]}", flush=True)
+    fmm = make_build(X, op, kappa=kappa, tol=1e-6, leaf_size=leaf, geometry=geom)
+    dev = SkelFMMBatched(fmm.tree, op, kappa=kappa).build_from_copy(fmm, use_single_precision=single)
+    rs = np.random.RandomState(2)
+    q = rs.randn(X.shape[0])
+    if op.is_complex:
+        q = q + 1j * rs.randn(X.shape[0])
+    x = torch.as_tensor(q).to(_gpu())
+    y0 = dev.matvec(x).cpu().numpy()
+    pairs0, ev0 = dev.sym_pairs, dev.evaluations
+    dev.set_self_block(block)
+    assert dev.self_block == block and dev.sym_pairs > pairs0 and dev.evaluations < ev0
+    assert dev.one_interactions + 2 * dev.sym_pairs == dev.interactions
+    y1 = dev.matvec(x).cpu().numpy()
+    err = rel_l2(y1, y0)
+    print(f"self blocks of {block} vs one-sided self segments: {err:.2e}")
+    assert err <= (3e-6 if single else 1e-13)
+    dev.set_near_field_split(1024)  # shorter source runs over the same tables
+    assert rel_l2(dev.matvec(x).cpu().numpy(), y1) <= (1e-6 if single else 1e-14)
+    dev.set_self_block(0)
+    assert dev.sym_pairs == pairs0
+    assert rel_l2(dev.matvec(x).cpu().numpy(), y0) <= (1e-6 if single else 1e-14)
+    if name == "laplace_3d_cube" and block == 64:
+        best = autotune.tune(dev, 1, use_cache=False, splits=(4096,), levels=False)
+        assert best["self_block"] in (0,) + tuple(autotune.SELF_BLOCKS) and dev.self_block == best["self_block"]
+        assert rel_l2(dev.matvec(x).cpu().numpy(), y0) <= (3e-6 if single else 1e-13)
+    with pytest.raises(ValueError):
+        dev.set_self_block(-1)
+
+
 @pytest.mark.parametrize("single,nrhs", [(False, 1), (True, 1), (False, 1j), (True, 20)])
 def test_level_variants_keep_parity(single, nrhs):
     """Every launch variant of the interpolation stages (loads in flight for one channel, matrix

@@ -170,6 +170,38 @@ def test_symmetric_split_emulation_matches_one_sided(case):
     assert rel_l2(y1, y0) < 1e-13
 
 
+@pytest.mark.parametrize("block", [16, 24])
+@pytest.mark.parametrize("case", [c for c in SYMMETRIC_BUILDS if c[0] in ("helm3d", "hadl")] + SYMMETRIC_BUILDS[:1],
+                         ids=lambda c: c[0])
+def test_self_blocks_emulate_one_sided(case, block):
+    """Self segments cut into blocks (pairs of different blocks symmetric, diagonal blocks one-sided)
+    give the same apply, the same interactions and tables that pass the layout check."""
+    from skelfmm_amd.layout import check_layout, split_self_blocks
+
+    _, f, k = case
+    L = build_layout(f, max_sources=900)
+    one, sym = L.split
+    seg = one.e_n[(one.e_flags & SRC_SELF) != 0]
+    assert np.any(seg > block), "no self segment to cut"
+    assert split_self_blocks(one, sym, L.max_sources, 0) == (one, sym)
+    one2, sym2 = split_self_blocks(one, sym, L.max_sources, block)
+    assert one2.interactions + 2 * sym2.interactions == L.interactions
+    # moved: the pairs of different blocks of every segment longer than the block
+    nb = (seg + block - 1) // block
+    moved = sum(int(block * (n - block * (i + 1))) for n, b in zip(seg, nb) if n > block for i in range(b - 1))
+    assert sym2.interactions == sym.interactions + moved
+    assert not np.any(sym2.e_flags & SRC_SELF)
+    assert np.all(one2.e_n[(one2.e_flags & SRC_SELF) != 0] <= block)
+    L.split = (one2, sym2)
+    L.evaluations = one2.interactions + sym2.interactions
+    check_layout(L)
+    q = np.random.RandomState(4).randn(f.N)
+    if f.operator.is_complex:
+        q = q + 1j * np.random.RandomState(5).randn(f.N)
+    L0 = build_layout(f, max_sources=900)
+    assert rel_l2(emulate_apply(L, f.operator.name, k, q, split=True), emulate_apply(L0, f.operator.name, k, q)) < 1e-13
+
+
 def test_pair_rows_partition():
     """Every row is one-sided, a pair representative, or the partner of exactly one representative."""
     from skelfmm_amd.layout import pair_rows
