@@ -202,6 +202,44 @@ def test_self_blocks_emulate_one_sided(case, block):
     assert rel_l2(emulate_apply(L, f.operator.name, k, q, split=True), emulate_apply(L0, f.operator.name, k, q)) < 1e-13
 
 
+def test_self_blocks_tables_by_hand():
+    """A 130-point self segment in blocks of 64: diagonal blocks (64, 64, 2) one-sided with the
+    self flag, block pairs (64 x 66 and 64 x 2) symmetric; the box's other entry and a self-flagged
+    entry whose charges sit elsewhere stay as they were."""
+    from skelfmm_amd.layout import NearLists, _empty_lists, make_items, split_self_blocks
+
+    one = NearLists()
+    one.tpi = 64
+    i32 = lambda *v: np.array(v, dtype=np.int32)
+    one.g_pt, one.g_n, one.g_out, one.g_box, one.g_kind = i32(100, 300), i32(130, 70), i32(100, 1300), i32(3, 5), i32(0, 1)
+    one.g_e0, one.g_e1 = np.array([0, 2], np.int64), np.array([2, 3], np.int64)
+    # entries: box 3's own points (self), box 7, and box 5's skeleton against its own points with
+    # the charges of the other kind (e_q != g_out: not cut)
+    one.e_pt, one.e_n, one.e_q = i32(100, 500, 300), i32(130, 40, 70), i32(100, 500, 300)
+    one.e_qkind, one.e_box, one.e_flags = i32(0, 0, 0), i32(3, 7, 5), i32(SRC_SELF, 0, SRC_SELF | SRC_NEG)
+    one.items, one.item_cost = make_items(one.g_n, one.g_e0, one.g_e1, one.e_n, 1000, 64)
+    one.interactions = 130 * 130 + 130 * 40 + 70 * 70
+    sym = _empty_lists(512)
+    o2, s2 = split_self_blocks(one, sym, 1000, 64)
+    grp = lambda P: [tuple(int(getattr(P, k)[g]) for k in ("g_pt", "g_n", "g_out", "g_box", "g_kind", "g_e0", "g_e1"))
+                     for g in range(P.g_pt.shape[0])]
+    ent = lambda P: [tuple(int(getattr(P, k)[s]) for k in ("e_pt", "e_n", "e_q", "e_box", "e_flags"))
+                     for s in range(P.e_pt.shape[0])]
+    assert grp(o2) == [(100, 130, 100, 3, 0, 0, 1), (300, 70, 1300, 5, 1, 1, 2), (100, 64, 100, 3, 0, 2, 3),
+                       (164, 64, 164, 3, 0, 3, 4), (228, 2, 228, 3, 0, 4, 5)]
+    assert ent(o2) == [(500, 40, 500, 7, 0), (300, 70, 300, 5, SRC_SELF | SRC_NEG), (100, 64, 100, 3, SRC_SELF),
+                       (164, 64, 164, 3, SRC_SELF), (228, 2, 228, 3, SRC_SELF)]
+    assert grp(s2) == [(100, 64, 100, 3, 0, 0, 1), (164, 64, 164, 3, 0, 1, 2)]
+    assert ent(s2) == [(164, 66, 164, 3, 0), (228, 2, 228, 3, 0)]
+    assert o2.interactions == 130 * 40 + 70 * 70 + 64 * 64 * 2 + 2 * 2 and s2.interactions == 64 * 66 + 64 * 2
+    assert o2.interactions + 2 * s2.interactions == one.interactions
+    # every item inside its group; blocks at least as long as the segment change nothing
+    for P in (o2, s2):
+        it = P.items.astype(np.int64)
+        assert np.all(it[:, 2] >= P.g_e0[it[:, 0]]) and np.all(it[:, 3] <= P.g_e1[it[:, 0]])
+    assert split_self_blocks(one, sym, 1000, 130) == (one, sym)
+
+
 def test_pair_rows_partition():
     """Every row is one-sided, a pair representative, or the partner of exactly one representative."""
     from skelfmm_amd.layout import pair_rows
