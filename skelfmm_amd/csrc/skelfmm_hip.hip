@@ -67,8 +67,9 @@ namespace {
 // projection rn linear in the normal (source normal: (x - y) . n_y; target normal: (x - y) . n_x),
 // so g is evaluated once (kernel_value at rn = 1) and scaled by the two directions' projections.
 // A work item holds up to kSymTargets targets of one group, taken in passes
-// of 64 * TPL (TPL per lane, in registers, with their own charges q[g_out + t]), and a run of
-// its source entries; the
+// of 64 * TPL (TPL per lane, in registers, with their own charges q[g_out + t]; a last pass of
+// 1-32 or 65-96 targets in rows of 32 that both halves of a wave hold, each half taking its own
+// sources), and a run of its source entries; the
 // 4 waves split every 256-source tile. Forward sums stay in the lane (FP64 across tiles, combined
 // over the waves through LDS at the end). Reverse sums: every lane writes its partial for each
 // source of a block of SUB sources to a per-wave LDS table; the wave then sums the table's
@@ -173,7 +174,10 @@ struct SymEnt {
 };
 
 // One pass of the symmetric kernel over targets tp .. tp + 64 TPL - 1 of the item (TPL per lane).
-template <typename R, int OP, bool PREC, int UNR, int TPL, bool FT>
+// HALF: rows of 32 targets, tp .. tp + 32 TPL - 1, held by both halves of the wave; each half takes
+// its own SUB sources of every block of 2 SUB (a last pass that would leave more than half of a
+// 64-lane row idle).
+template <typename R, int OP, bool PREC, int UNR, int TPL, bool FT, bool HALF = false>
 __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, double* __restrict__ gtile,
                                          R* __restrict__ rvl, double* __restrict__ red, int* __restrict__ sidx,
                                          const SymEnt<R>* et, bool etab, int nent, int tot, int tp, int tend, int t0,
@@ -188,6 +192,9 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
   constexpr int LPS = SymSub<R, CW>::lanes;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  constexpr int TROW = HALF ? 32 : 64;                     // targets per row
+  const int tl = HALF ? (lane & 31) : lane;                // this lane's target in the row
+  const int hoff = HALF ? (lane >> 5) * SUB : 0;           // this lane's sources in a block
   const int g = reinterpret_cast<const int4*>(d.items)[blockIdx.x].x;
   const int gpt = d.g_pt[g], gout = d.g_out[g];
   const R* px = static_cast<const R*>(d.px);
@@ -197,13 +204,13 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
   R* uo = static_cast<R*>(d.u);
   const int w = d.width;
   const R kappa = kernel_kappa<R, FT>(d.kappa);
-  // targets tp + lane + 64 k; padding targets repeat the first one with zero charge
+  // targets tp + tl + TROW k; padding targets repeat the first one with zero charge
   R qt[TPL * CW], xt[TPL], yt[TPL], zt[TPL];
   R nt[NRM ? TPL : 1][3];            // plain form of a double layer: target normals
   double gt[G64 ? TPL : 1][kGeoRow];  // precision form of a double layer: target FP64 rows
 #pragma unroll
   for (int k = 0; k < TPL; ++k) {
-    const int t = tp + lane + 64 * k;
+    const int t = tp + tl + TROW * k;
     const int sl = gpt + (t < tend ? t : t0);
     xt[k] = px[sl];
     yt[k] = py[sl];
@@ -234,12 +241,14 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
     const double* gw = G64 ? &gtile[wave * 64 * kGeoRow] : nullptr;
 #pragma unroll
     for (int a = 0; a < TPL * CW; ++a) part[a] = 0;
-    for (int sub = 0; sub < cnt; sub += SUB) {
-      // only the block's valid rows are evaluated; table rows past them are never summed
+    for (int sub = 0; sub < cnt; sub += (HALF ? 2 : 1) * SUB) {
+      // only the block's valid rows are evaluated; table rows past them are never summed (half
+      // rows: the second half evaluates rows sub + SUB + jj, zero-charge padding past the tile's
+      // sources where those end first)
       const int nj = min(SUB, cnt - sub);
 #pragma unroll UNR
       for (int jj = 0; jj < nj; ++jj) {
-        const R* row = tw + (sub + jj) * NV;
+        const R* row = tw + (sub + hoff + jj) * NV;
         const R xs = row[0];
         const R ys = row[1];
         const R zs = (S::dim == 3) ? row[2] : R(0);
@@ -256,7 +265,7 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
           if constexpr (GEOM != 0) {
             R rf, rr;
             if constexpr (G64) {
-              const double* gs = gw + (sub + jj) * kGeoRow;
+              const double* gs = gw + (sub + hoff + jj) * kGeoRow;
               rf = Geo64<OP>::rn(gt[k] + S::toff, gs + S::soff);
               rr = Geo64<OP>::rn(gs + S::toff, gt[k] + S::soff);
             } else {
@@ -288,15 +297,18 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
         if (CW == 2) rvl[((wave * SUB + jj) * CW + CW - 1) * 64 + lane] = r1;
       }
       wave_sync_lds();
-      // column sums: lanes j * LPS .. j * LPS + LPS - 1 hold source j's partial over 64 / LPS lanes
-      const int j = lane / LPS, pp = lane % LPS;
+      // column sums: lanes j * LH .. j * LH + LH - 1 sum source j's partials, PER each (half
+      // rows: source j < SUB is row j, lanes 0-31; source SUB + j row j, lanes 32-63)
+      constexpr int LH = HALF ? LPS / 2 : LPS;
       constexpr int PER = 64 / LPS;
+      const int j = lane / LH, pp = lane % LH;
+      const int trow = HALF ? j % SUB : j, tcol = HALF ? (j / SUB) * 32 : 0;
 #pragma unroll
       for (int c = 0; c < CW; ++c) {
         R v = 0;
 #pragma unroll
-        for (int i = 0; i < PER; ++i) v += rvl[((wave * SUB + j) * CW + c) * 64 + pp * PER + i];
-        v = group_sum<LPS>(v);
+        for (int i = 0; i < PER; ++i) v += rvl[((wave * SUB + trow) * CW + c) * 64 + tcol + pp * PER + i];
+        v = group_sum<LH>(v);
         if (pp == 0 && sub + j < cnt) dst(wb + sub + j, c, double(v));
       }
       wave_sync_lds();
@@ -402,18 +414,21 @@ __device__ __forceinline__ void sym_pass(const P2PDev& d, R* __restrict__ tile, 
       }
     }
   }
-  // forward sums: combine the 4 waves, one target slot at a time
+  // forward sums: combine the 4 waves (and the two halves of half rows), one target slot at a time
 #pragma unroll
   for (int k = 0; k < TPL; ++k) {
 #pragma unroll
     for (int c = 0; c < CW; ++c) red[(wave * kTCH + lane) * CW + c] = acc[k * CW + c];
     __syncthreads();
-    const int t = tp + lane + 64 * k;
-    if (wave == 0 && t < tend) {
+    const int t = tp + tl + TROW * k;
+    if (wave == 0 && lane < TROW && t < tend) {
 #pragma unroll
       for (int c = 0; c < CW; ++c) {
-        const double v = red[(0 * kTCH + lane) * CW + c] + red[(1 * kTCH + lane) * CW + c] + red[(2 * kTCH + lane) * CW + c] +
-                         red[(3 * kTCH + lane) * CW + c];
+        double v = red[(0 * kTCH + lane) * CW + c] + red[(1 * kTCH + lane) * CW + c] + red[(2 * kTCH + lane) * CW + c] +
+                   red[(3 * kTCH + lane) * CW + c];
+        if (HALF)
+          v += red[(0 * kTCH + lane + 32) * CW + c] + red[(1 * kTCH + lane + 32) * CW + c] +
+               red[(2 * kTCH + lane + 32) * CW + c] + red[(3 * kTCH + lane + 32) * CW + c];
         atomic_acc(&uo[(long long)(gout + t) * w + c], R(v));
       }
     }
@@ -496,9 +511,19 @@ __global__ __launch_bounds__(kWG) void p2p_sym_kernel(P2PDev d) {
   }
   const int tend = min(ng, t0 + kSymTargets);
   // wave-uniform passes; the last one takes as many targets per lane as it needs (no idle slots
-  // beyond the next multiple of 64)
+  // beyond the next multiple of 64), in rows of 32 where those pad less (1 or 3 per lane)
   for (int tp = t0; tp < tend;) {
     const int tpl = min(TPL, (tend - tp + 63) / 64);
+    const int tph = (tend - tp + 31) / 32;
+    if (tph == 1 || (TPL >= 3 && tph == 3)) {  // then 32 tph < 64 tpl: the last pass
+      if (tph == 1)
+        sym_pass<R, OP, PREC, UNR, 1, FT, true>(d, tile, gtile, rvl, red, sidx, &et, etab, nent, tot, tp, tend, t0, sb,
+                                                se, ctx, cty, ctz);
+      else
+        sym_pass<R, OP, PREC, UNR, (TPL >= 3 ? 3 : 1), FT, true>(d, tile, gtile, rvl, red, sidx, &et, etab, nent, tot,
+                                                                 tp, tend, t0, sb, se, ctx, cty, ctz);
+      break;
+    }
     if (TPL >= 4 && tpl >= 4)
       sym_pass<R, OP, PREC, UNR, (TPL >= 4 ? 4 : 1), FT>(d, tile, gtile, rvl, red, sidx, &et, etab, nent, tot, tp, tend,
                                                           t0, sb, se, ctx, cty, ctz);

@@ -148,12 +148,13 @@ def _case3(nt, ne, seed=0):
 
 
 @pytest.mark.parametrize("precision,sym_unroll", [(0, 2), (1, 1), (1, 2), (1, 4)])
-@pytest.mark.parametrize("nt,ne", [(40, 100), (600, 30), (97, 300), (1, 5)])
+@pytest.mark.parametrize("nt,ne", [(40, 100), (600, 30), (97, 300), (1, 5), (20, 60), (70, 200), (280, 40)])
 def test_symmetric_laplace3d_unrolls(precision, sym_unroll, nt, ne):
     """3D Laplace symmetric pairs: every inner-loop unroll of the FP32 form and the FP64 form
     against an FP64 sum, on one group of nt targets (one or two work items, passes of 256
-    targets with partial lanes) and ne entries of several points (up to 128 entries in the FP32
-    form: sources streamed across entries; beyond, and in the FP64 form: entry by entry)."""
+    targets with partial lanes; last passes of 1, 20, 70, 88 and 24 targets in rows of 32 held by
+    both wave halves) and ne entries of several points (up to 128 entries in the FP32 form:
+    sources streamed across entries; beyond, and in the FP64 form: entry by entry)."""
     X, centers, box, sign, e_n = _case3(nt, ne)
     q = np.random.default_rng(3).normal(size=(X.shape[0], 1))
     d = X[:nt, None, :] - X[None, nt:, :]
@@ -173,7 +174,7 @@ def test_symmetric_laplace3d_unrolls(precision, sym_unroll, nt, ne):
 
 @pytest.mark.parametrize("kernel,name", [(4, "laplace_double_layer_3d"), (5, "laplace_adjoint_double_layer_3d")])
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("nt,ne", [(40, 100), (97, 300), (600, 30)])
+@pytest.mark.parametrize("nt,ne", [(40, 100), (97, 300), (600, 30), (20, 60), (280, 150)])
 def test_symmetric_double_layers(kernel, name, precision, nt, ne):
     """Double-layer pairs evaluated once for both directions: forward K(t, s), reverse K(s, t)
     (the normal on the other side), against the oracle's kernel matrices in FP64; 300 entries
