@@ -57,7 +57,7 @@ HBM_BYTES_PER_S = 8.0e12
 # same run reads), keyed by (N, tol, leaf), with the setting that was profiled (symmetric unroll,
 # source-run length, self block; the traffic barely depends on any of them);
 # profiles/apply_laplace3d_uniform1M_hbm_traffic.md
-P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400): {"bytes": 568_321_905 + 640_487_547, "sym_unroll": 2,
+P2P_FP32_TRAFFIC_BYTES = {(1_000_000, 1e-6, 400): {"bytes": 567_433_414 + 637_247_584, "sym_unroll": 2,
                                                    "max_sources": 4096, "self_block": 192}}
 
 
