@@ -150,10 +150,11 @@ def tune(dev, width=1, reps=3, use_cache=True, splits=SPLITS, levels=True):
             "near_field_ms": round(t, 4), "tried": [[round(a, 4), b, c, d, e] for a, b, c, d, e in results]}
     apply_setting(dev, best)
     if sym and hasattr(dev, "set_self_block"):
+        # the candidates differ by a few percent: each is timed twice, the faster run kept
         timed = []
         for blk in (0,) + tuple(SELF_BLOCKS):
             dev.set_self_block(blk)
-            timed.append((dev.time_near_field(width, reps=reps), blk))
+            timed.append((min(dev.time_near_field(width, reps=max(reps, 5)) for _ in range(2)), blk))
         t, blk = min(timed)
         best["self_block"] = blk
         best["near_field_ms"] = round(t, 4)
