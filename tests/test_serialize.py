@@ -97,6 +97,33 @@ def test_roundtrip_apply_is_identical(case):
     assert rel_l2(y2, y0) < 1e-13
 
 
+def test_roundtrip_of_self_blocks():
+    """A layout whose self segments were cut into symmetric blocks saves and loads like any other:
+    the loaded tables pass the layout check, apply identically, and cut again (block 0 and a
+    block at least as long as the diagonal blocks) change nothing."""
+    from skelfmm_amd.layout import split_self_blocks
+
+    _, f, kappa, geom = CASES[0]
+    L = build_layout(f, max_sources=2048)
+    L.split = split_self_blocks(*L.split, L.max_sources, 32)
+    L.evaluations = L.split[0].interactions + L.split[1].interactions
+    buf = io.BytesIO()
+    save_layout(buf, L, operator_name=f.operator.name, kappa=kappa, tol=f.tol, leaf_size=f.tree.leaf_size,
+                points=f.tree.XX)
+    buf.seek(0)
+    M = load_layout(buf).layout
+    for P, Q in zip(L.split, M.split):
+        assert P.tpi == Q.tpi and P.interactions == Q.interactions
+        for k in P.FIELDS + ("items", "item_cost"):
+            _same(getattr(P, k), getattr(Q, k), "split." + k)
+    q = np.random.RandomState(1).randn(f.N)
+    y0 = emulate_apply(L, f.operator.name, kappa, q, split=True)
+    assert np.array_equal(emulate_apply(M, f.operator.name, kappa, q, split=True), y0)
+    assert split_self_blocks(*M.split, M.max_sources, 0) == M.split
+    assert split_self_blocks(*M.split, M.max_sources, 32) == M.split
+    assert rel_l2(y0, emulate_apply(M, f.operator.name, kappa, q)) < 1e-13
+
+
 def _rewrite(buf, mutate_header=None, mutate_arrays=None):
     buf.seek(0)
     with np.load(buf, allow_pickle=False) as z:
