@@ -147,9 +147,9 @@ typedef struct skf_p2p_args {
      skf_jit_compile) for channel blocks of 1, 2, 4 and 8, in this argument's precision */
   void* user_fn[4];
   /* an upper bound of every pair distance (e.g. the point cloud's diameter), 0 = unknown; with
-     it the 3D Helmholtz kernels evaluate e^{i kappa r} with a branch-free reduced sine/cosine
-     (FP32: hardware sine/cosine after the reduction, FP64: polynomials) when
-     kappa * max_distance <= 65536 */
+     it the 3D Helmholtz kernels evaluate e^{i kappa r} with a branch-free sine/cosine (FP32:
+     the hardware sine/cosine of the phase in revolutions when kappa * max_distance <= 1600;
+     FP64: reduction and polynomials when kappa * max_distance <= 65536) */
   double max_distance;
   /* precision form of the double-layer kernels (required there, ignored otherwise): FP64 rows
      [7][geo64_n] per slot: absolute x, y, z, normal nx, ny, nz, and n . x. The normal projection

@@ -59,22 +59,23 @@ struct Mth<double> {
   __device__ static inline double by0(double x) { return y0(x); }
 };
 
-// FP32 sine and cosine of a bounded phase, without a branch. The host checks that every kernel
-// argument kappa r is at most kTrigFastMax and then selects these instances; their kernel
-// constant is kappa / (2 pi) (kernel_kappa), so the phase comes out in revolutions,
+// FP32 sine and cosine of a bounded phase, without a branch or a reduction. The host checks that
+// every kernel argument kappa r is at most kTrigDirectMax and then selects these instances; their
+// kernel constant is kappa / (2 pi) (kernel_kappa), so the phase comes out in revolutions,
 // t = r kappa / (2 pi): one rounding of the constant and one of the product, as forming kappa r
-// has. t - rint(t) is exact (|t| < 2^22: the integer is 0, or both operands are FP32 within a
-// factor of 2 of each other), and the hardware v_sin_f32 / v_cos_f32 take revolutions in
-// [-1/2, 1/2] (measured max absolute error 1.3e-7 there against FP64 on an MI355X). Other
-// arguments use sincosf.
+// has. The hardware v_sin_f32 / v_cos_f32 take revolutions and reduce them exactly themselves
+// within their input range |t| <= 256 (kTrigDirectMax keeps |t| below 255): measured max absolute
+// error 1.25e-7 against FP64 on an MI355X for every |t| up to 300, the same as on [-1/2, 1/2].
+// Other arguments use sincosf. The FP64 instances (sincos_reduced) take kappa r up to
+// kTrigFastMax.
 constexpr float kTrigFastMax = 65536.0f;
+constexpr float kTrigDirectMax = 1600.0f;
 constexpr double kInvTwoPi = 0.15915494309189535;  // 1 / (2 pi)
 constexpr float kTwoPiF = 6.28318530717958648f;
 
 __device__ inline void sincos_rev(float t, float* sp, float* cp) {
-  const float y = t - rintf(t);
-  *sp = __builtin_amdgcn_sinf(y);
-  *cp = __builtin_amdgcn_cosf(y);
+  *sp = __builtin_amdgcn_sinf(t);
+  *cp = __builtin_amdgcn_cosf(t);
 }
 
 // The same for FP64 (the plain form), from the mathematics alone: Cody-Waite reduction
@@ -136,7 +137,8 @@ template <> struct OpT<SKF_HELMHOLTZ_DL_3D> { static constexpr int dim = 3, cplx
 template <> struct OpT<SKF_HELMHOLTZ_ADL_3D> { static constexpr int dim = 3, cplx = 1, geom = 2; };
 
 // K(x, y) for dx = x - y and the normal dot product rn = dx . n (source or target normal).
-// FT: the Helmholtz arguments are bounded by kTrigFastMax (the host checks it and selects these
+// FT: the Helmholtz arguments are bounded by kTrigDirectMax (FP32) or kTrigFastMax (FP64) (the
+// host checks it and selects these
 // instances)
 template <typename R, bool FT>
 __device__ inline void sincos_k(R x, R* s, R* c) {
@@ -257,7 +259,7 @@ struct P2PDev {
   int nch;  // channels (complex channels for complex kernels)
   double kappa;
   double* urev;  // reverse outputs of the symmetric kernel (FP64; u itself in the FP64 form)
-  int fast_trig;  // every kernel argument kappa r is within kTrigFastMax
+  int fast_trig;  // every kernel argument kappa r is within kTrigDirectMax (FP32) / kTrigFastMax (FP64)
   // precision form of the double layers: FP64 rows [7][geo64_n] (x, y, z, nx, ny, nz, n.x)
   const double* geo64;
   long long geo64_n;

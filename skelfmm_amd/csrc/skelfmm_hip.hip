@@ -1516,7 +1516,8 @@ int run_p2p(const skf_p2p_args* a, const void* q, void* u, hipStream_t st) {
   // the 3D Helmholtz kernels take the branch-free reduced sincos (FP32: hardware sine/cosine after
   // the reduction; FP64: polynomials) when the host bounds every argument kappa r (kappa times
   // the largest pair distance) within its range
-  d.fast_trig = (a->max_distance > 0.0 && std::fabs(a->kappa) * a->max_distance <= double(kTrigFastMax)) ? 1 : 0;
+  const double trig_max = a->precision == 1 ? double(kTrigDirectMax) : double(kTrigFastMax);
+  d.fast_trig = (a->max_distance > 0.0 && std::fabs(a->kappa) * a->max_distance <= trig_max) ? 1 : 0;
   const int mfma_min = a->mfma_min_channels == 0 ? 16 : a->mfma_min_channels;
   if (a->precision != 0 && a->precision != 1) return set_err(SKF_ERR_ARG, "precision must be 0 or 1");
   if (op_user(a->kernel)) return launch_user(a, d, a->nitems, st);

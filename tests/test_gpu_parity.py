@@ -133,8 +133,8 @@ def test_precision_form_against_direct_sum():
 
 
 def test_fp32_helmholtz_fast_trig_accuracy():
-    """The FP32 form's Helmholtz near field with the branch-free reduction and the hardware
-    sine/cosine (taken when kappa times the cloud diameter <= 65536) against the same build with
+    """The FP32 form's Helmholtz near field with the hardware sine/cosine of the phase in
+    revolutions (taken when kappa times the cloud diameter <= 1600) against the same build with
     the library's sincosf (max_distance 0 turns the fast path off) and the FP64 apply, at kappa
     40 (kernel arguments up to ~50 in the near field)."""
     from skelfmm_amd.batched import SkelFMMBatched
@@ -146,7 +146,7 @@ def test_fp32_helmholtz_fast_trig_accuracy():
     x = torch.as_tensor(q).to(_gpu())
     y64 = SkelFMMBatched(fmm.tree, fmm.operator, kappa=fmm.kappa).build_from_copy(fmm).matvec(x).cpu().numpy()
     d32 = SkelFMMBatched(fmm.tree, fmm.operator, kappa=fmm.kappa).build_from_copy(fmm, use_single_precision=True)
-    assert 0 < 40.0 * d32.max_distance <= 65536
+    assert 0 < 40.0 * d32.max_distance <= 1600
     x32 = torch.as_tensor(q.astype(np.complex64)).to(_gpu())
     y_fast = d32.matvec(x32).cpu().numpy().astype(np.complex128)
     d32.max_distance = 0.0

@@ -1,14 +1,14 @@
 """The device sine/cosine of the Helmholtz kernels (``csrc/p2p_core.h``).
 
-FP32 (``sincos_rev``): instances whose arguments the host bounds (kappa r <= kTrigFastMax) take
-kappa / (2 pi) as their constant, form the phase in revolutions t = r kappa / (2 pi), reduce it
-by t - rint(t) and hand it to the hardware sine and cosine (v_sin_f32 / v_cos_f32 take
-revolutions; measured max absolute error 1.3e-7 over [-1/2, 1/2] against FP64 on an MI355X,
-checked end to end on the GPU by ``test_gpu_parity.py::test_fp32_helmholtz_fast_trig_accuracy``).
-Here, in numpy FP32: the constants are 1/(2 pi) and 2 pi; the reduction is exact over the whole
-range the host allows; and the phase error of the whole path (constant and product each rounded
-once) stays within two roundings of the exact kappa r, as forming kappa r in FP32 does (a
-constant off by one FP32 ulp already fails the phase bound at the top of the range).
+FP32 (``sincos_rev``): instances whose arguments the host bounds (kappa r <= kTrigDirectMax) take
+kappa / (2 pi) as their constant, form the phase in revolutions t = r kappa / (2 pi) and hand it
+to the hardware sine and cosine, which take revolutions and reduce them exactly within their
+input range |t| <= 256 (v_sin_f32 / v_cos_f32; measured max absolute error 1.25e-7 against FP64
+on an MI355X for every |t| up to 300, checked end to end on the GPU by
+``test_gpu_parity.py::test_fp32_helmholtz_fast_trig_accuracy``). Here, in numpy FP32: the
+constants are 1/(2 pi) and 2 pi; the host's bound keeps |t| inside the hardware range; and the
+phase error of the whole path (constant and product each rounded once) stays within two
+roundings of the exact kappa r, as forming kappa r in FP32 does.
 
 FP64 (``sincos_reduced(double)``): Cody-Waite reduction by pi/2 and Taylor polynomials, run with
 exact FMAs against libm.
@@ -22,7 +22,8 @@ import numpy as np
 
 HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "skelfmm_amd", "csrc",
                       "p2p_core.h")
-TRIG_FAST_MAX = 65536.0
+TRIG_FAST_MAX = 65536.0  # FP64 instances
+TRIG_DIRECT_MAX = 1600.0  # FP32 instances
 
 
 def _const(name):
@@ -33,11 +34,10 @@ def _const(name):
 
 
 def _phase_rev(r, kappa):
-    """The device path in FP32: constant kappa / (2 pi) rounded once, product rounded once,
-    reduction t - rint(t); returns (t, reduced revolutions)."""
+    """The device path in FP32: constant kappa / (2 pi) rounded once, product rounded once;
+    returns the phase in revolutions (the hardware sine/cosine reduce it themselves)."""
     krev = np.float32(np.float64(kappa) * _const("kInvTwoPi"))
-    t = np.float32(np.float32(r) * krev)
-    return t, np.float32(t - np.float32(np.rint(t)))
+    return np.float32(np.float32(r) * krev)
 
 
 def test_constants_are_the_mathematical_ones():
@@ -46,17 +46,18 @@ def test_constants_are_the_mathematical_ones():
     assert abs(Decimal(_const("kInvTwoPi")) - 1 / (2 * pi)) < Decimal("1e-17")
     assert _const("kTwoPiF") == float(2 * pi) and np.float32(_const("kTwoPiF")) == np.float32(2 * np.pi)
     assert _const("kTrigFastMax") == TRIG_FAST_MAX
+    assert _const("kTrigDirectMax") == TRIG_DIRECT_MAX
 
 
-def test_reduction_is_exact_over_the_allowed_range():
+def test_phase_stays_inside_the_hardware_range():
+    """At the host's bound every phase the FP32 path forms, roundings included, is below 256
+    revolutions (the input range of v_sin_f32 / v_cos_f32), for any kappa."""
     rng = np.random.default_rng(0)
-    tmax = TRIG_FAST_MAX / (2 * np.pi)
-    ts = np.concatenate([rng.uniform(0.0, 2.0, 4000), rng.uniform(0.0, tmax, 4000),
-                         np.arange(0, 41) * 0.25, [0.0, 1e-30, 0.5, 1.5, 2.5, tmax]]).astype(np.float32)
-    y = ts - np.rint(ts)  # FP32 arithmetic
-    exact = ts.astype(np.float64) - np.rint(ts.astype(np.float64))
-    assert np.array_equal(y.astype(np.float64), exact)
-    assert np.all(np.abs(y) <= 0.5)
+    for kappa in (1e-3, 1.0, 10 * np.pi, 40.0, 1600.0, 1e5):
+        rmax = np.float32(TRIG_DIRECT_MAX / kappa)
+        r = np.concatenate([rng.uniform(0.0, rmax, 2000), [rmax, np.nextafter(rmax, np.float32(0))]]).astype(np.float32)
+        t = _phase_rev(r, kappa)
+        assert np.all(np.abs(t) < 255.0), float(np.max(np.abs(t)))
 
 
 def test_phase_error_is_two_roundings_of_kappa_r():
@@ -66,9 +67,9 @@ def test_phase_error_is_two_roundings_of_kappa_r():
     rng = np.random.default_rng(1)
     worst_new = worst_ref = 0.0
     for kappa in (1.0, 10 * np.pi, 20.0, 40.0, 1000.0):
-        rmax = TRIG_FAST_MAX / kappa
+        rmax = TRIG_DIRECT_MAX / kappa
         r = np.concatenate([rng.uniform(0.0, min(rmax, 1.0), 2000), rng.uniform(0.0, rmax, 2000)]).astype(np.float32)
-        t, y = _phase_rev(r, kappa)
+        y = _phase_rev(r, kappa)
         exact = (r.astype(np.float64) * np.float64(np.float32(kappa))) / (2 * np.pi)  # revolutions
         err = np.abs(((y.astype(np.float64) - exact) + 0.5) % 1.0 - 0.5) * 2 * np.pi  # radians
         bound = 3 * 2.0**-24 * exact * 2 * np.pi + 1e-12
@@ -77,7 +78,7 @@ def test_phase_error_is_two_roundings_of_kappa_r():
         err_ref = np.abs(((kr.astype(np.float64) / (2 * np.pi) - exact) + 0.5) % 1.0 - 0.5) * 2 * np.pi
         worst_new = max(worst_new, float(np.max(err / np.maximum(exact, 1.0))))
         worst_ref = max(worst_ref, float(np.max(err_ref / np.maximum(exact, 1.0))))
-        # with an ideal sine of the reduced revolutions, sin and cos follow the phase
+        # with an ideal sine of the revolutions, sin and cos follow the phase
         assert np.max(np.abs(np.sin(2 * np.pi * y.astype(np.float64)) - np.sin(2 * np.pi * exact))) <= np.max(bound) + 1e-9
     assert worst_new <= 2.0 * worst_ref + 1e-12, (worst_new, worst_ref)
 
