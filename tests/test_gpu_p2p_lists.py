@@ -40,7 +40,7 @@ def _case(seed=0, ne=NE):
 
 
 def _run(precision, width, X, centers, box, sign, q, symmetric=False, kernel=0, nt=NT, e_n=None, sym_unroll=0,
-         normals=None):
+         normals=None, kappa=0.0, max_distance=0.0):
     NE = sign.shape[0]
     dim = X.shape[1]
     e_n = np.ones(NE, np.int64) if e_n is None else np.asarray(e_n, np.int64)
@@ -82,7 +82,8 @@ def _run(precision, width, X, centers, box, sign, q, symmetric=False, kernel=0, 
         keep = [one, sym, urev]
         a.one, a.sym = ctypes.pointer(one), ctypes.pointer(sym)
         a.urev, a.urev_len = (hip.ptr(urev), urev.numel()) if precision == 1 else (None, 0)
-    a.kernel, a.precision, a.width, a.kappa = kernel, precision, width, 0.0
+    a.kernel, a.precision, a.width, a.kappa = kernel, precision, width, kappa
+    a.max_distance = max_distance
     a.px, a.py, a.pz = hip.ptr(px), hip.ptr(py), (hip.ptr(pz) if pz is not None else None)
     a.sym_unroll = sym_unroll
     if normals is not None:
@@ -206,3 +207,34 @@ def test_symmetric_double_layers(kernel, name, precision, nt, ne):
     print(f"  rel-l2 targets {err_t:.2e}, sources {err_s:.2e}", flush=True)
     bar = 1e-13 if precision == 0 else 2e-6
     assert err_t < bar and err_s < bar
+
+
+@pytest.mark.parametrize("kappa", [40.0, 800.0])
+def test_fp32_helmholtz_phase_over_the_hardware_range(kappa):
+    """3D Helmholtz pairs in the FP32 form with the hardware sine/cosine of the unreduced phase in
+    revolutions (t = kappa r / 2 pi up to about 220 at kappa 800, inside the hardware's |t| <= 256)
+    against the same lists with sincosf (max_distance 0 turns the fast path off), both against an
+    FP64 sum of e^{i kappa r} / r: the fast path loses nothing beyond the FP32 rounding of the
+    phase that both paths share."""
+    nt, ne = 200, 60
+    X, centers, box, sign, e_n = _case3(nt, ne, seed=9)
+    D = float(np.linalg.norm(X.max(axis=0) - X.min(axis=0)))
+    assert kappa * D <= 1600
+    rng = np.random.default_rng(10)
+    qc = rng.normal(size=X.shape[0]) + 1j * rng.normal(size=X.shape[0])
+    q = np.stack([qc.real, qc.imag], axis=1)
+    d = X[:nt, None, :] - X[None, nt:, :]
+    r = np.sqrt(np.sum(d * d, axis=-1))
+    K = np.exp(1j * kappa * r) / r * np.repeat(sign, e_n)[None, :]
+    ref = np.zeros(X.shape[0], np.complex128)
+    ref[:nt] = K @ qc[nt:]
+    ref[nt:] = K.T @ qc[:nt]
+    out = {}
+    for name, md in (("fast", D), ("sincosf", 0.0)):
+        u = _run(1, 2, X, centers, box, sign, q, symmetric=True, kernel=3, nt=nt, e_n=e_n, kappa=kappa,
+                 max_distance=md)
+        out[name] = np.linalg.norm(u[:, 0] + 1j * u[:, 1] - ref) / np.linalg.norm(ref)
+    print(f"kappa {kappa}: max phase {kappa * D / (2 * np.pi):.1f} revolutions, rel-l2 fast {out['fast']:.2e}, "
+          f"sincosf {out['sincosf']:.2e}", flush=True)
+    assert out["fast"] <= 1.5 * out["sincosf"] + 1e-7
+    assert out["fast"] < 4 * 2.0**-24 * kappa * D + 1e-6  # a few FP32 roundings of the largest phase
